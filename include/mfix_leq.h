@@ -1,0 +1,216 @@
+/*
+ * mfix_leq.h - C ABI of the MI355X (gfx950) BiCGSTAB linear-equation solver
+ * that replaces MFIX's LEQ_BICGS hot path.
+ *
+ * Reference interfaces replaced (paths relative to the MFIX 2016-1 tree):
+ *   LEQ_BICGS(VNAME,VNO,VAR,A_M,B_M,CMETHOD,TOL,PC,ITMAX,IER)
+ *       model/leq_bicgs.f:23-90 (PC dispatch) and :107-509 (LEQ_BICGS0)
+ *   SOLVE_LIN_EQ(VNAME,VNO,VAR,A_M,B_M,M,ITMAX,METHOD,SWEEP,TOL1,PC,IER)
+ *       model/solve_lin_eq.f:17-195 (METHOD=2 branch, :151-152)
+ *   LEQ_MATVEC   model/leqsol_mod.f:103-265
+ *   LEQ_MSOLVE   model/leqsol_mod.f:286-598 (LINE; sweeps :767-1086, DGTSV)
+ *   LEQ_MSOLVE0  model/leqsol_mod.f:616-663 (NONE)
+ *   LEQ_MSOLVE1  model/leqsol_mod.f:681-749 (DIAG)
+ *   dot_product_par / dot_product_par2  model/leqsol_mod.f:1094-1295
+ *   grid descriptor (module globals c0,c1,c2, ranges, CELL_CLASS,
+ *   INCREMENT_FOR_MP)  model/dmp_modules/gridmap_mod.f:259-556,
+ *   model/set_increments.f:72-178
+ *
+ * Storage contract (identical to the reference's serial layout):
+ *   ni = IMAX+2, nj = JMAX+2, nk = KMAX+2 (nk = 1 for a 2-D run, NO_K),
+ *   DIMENSION_3 = ni*nj*nk, ijk (0-based) = (j-1) + (i-1)*nj + (k-1)*nj*ni
+ *   with 1-based i,j,k: J is the stride-1 axis.
+ *   A_M is DIMENSION_3 x 7 column-major: plane l (l = -3..3) starts at
+ *   element (l+3)*DIMENSION_3; l = -3 bottom (K-), -2 south (J-),
+ *   -1 west (I-), 0 diagonal, +1 east (I+), +2 north (J+), +3 top (K+).
+ *
+ * Return values: every function returns 0 on success or a negative
+ * MFIX_LEQ_E* code.  The reference's own solver status (IER = 0/-1/-2) is
+ * returned through the ier argument exactly as LEQ_BICGS sets it.
+ */
+#ifndef MFIX_LEQ_H
+#define MFIX_LEQ_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes of the C ABI (never confused with IER, which is <= 0) -- */
+#define MFIX_LEQ_OK 0
+#define MFIX_LEQ_EBADPC (-101)   /* PC not LINE/DIAG/NONE: reference calls mfix_exit (leq_bicgs.f:83-87) */
+#define MFIX_LEQ_ENOGRID (-102)  /* mfix_leq_grid_init not called / invalid grid */
+#define MFIX_LEQ_EHIP (-103)     /* a HIP runtime call failed */
+#define MFIX_LEQ_EINVAL (-104)   /* invalid argument */
+#define MFIX_LEQ_ENOMEM (-105)   /* device allocation failed / over budget */
+#define MFIX_LEQ_EDGTSV (-106)   /* DGTSV INFO>0 in a JK/IJ sweep: reference calls mfix_exit (leqsol_mod.f:991-996,1073-1078) */
+#define MFIX_LEQ_EBUSY (-107)    /* serving queue full */
+#define MFIX_LEQ_EMETHOD (-108)  /* SOLVE_LIN_EQ METHOD other than 2 (BiCGSTAB) */
+
+/* ---- grid descriptor: replaces the compar/geometry/indices module state -- */
+typedef struct mfix_leq_grid {
+    int imax, jmax, kmax;        /* interior cells per axis (IMAX, JMAX, KMAX) */
+    int no_k;                    /* 1 = two-dimensional run (NO_K); kmax ignored */
+    int cyclic_x, cyclic_y, cyclic_z; /* CYCLIC_X/Y/Z with NODES*=1 (set_increments.f:72-112) */
+    int is_serial;               /* IS_SERIAL (init_namelist.f:823): 1 = dot over all DIMENSION_3 cells */
+    int minimize_dotproducts;    /* MINIMIZE_DOTPRODUCTS (leqsol_mod.f:29) */
+    int icheck_bicgs;            /* ICHECK_BICGS (init_namelist.f:802), >= 1 */
+} mfix_leq_grid;
+
+/* Per-solve statistics (the reference exposes only IER and ITER_TOT). */
+typedef struct mfix_leq_stats {
+    int iterations;              /* BiCGSTAB iterations executed (loop trips) */
+    int iter_tot_increment;      /* what LEQ_BICGS0 adds to ITER_TOT(VNO) */
+    int ier;                     /* IER as LEQ_BICGS returns it */
+    int exit_reason;             /* 0 converged, 1 S-norm exit, 2 rho=0, 3 itmax, 4 zero row */
+    double rnorm0;               /* ||b - A x0|| of the scaled system */
+    double rnorm;                /* final recursive residual norm */
+    double true_rnorm;           /* ||b - A x|| recomputed in FP64 (mixed/refined paths; else -1) */
+    int refinements;             /* FP64 refinement cycles (mixed precision), 0 for FP64 solves */
+    int inner_iterations;        /* FP32 inner iterations summed over cycles (mixed) */
+    double device_ms;            /* device time of the solve measured with HIP events */
+} mfix_leq_stats;
+
+/* ---- lifetime ----------------------------------------------------------- */
+int mfix_leq_init(int device);                 /* select device, create stream */
+int mfix_leq_finalize(void);                   /* free every device buffer */
+int mfix_leq_grid_init(const mfix_leq_grid *g);/* replaces GRIDMAP_INIT+SET_INCREMENTS state */
+int64_t mfix_leq_dimension_3(void);            /* DIMENSION_3 of the current grid, or <0 */
+int mfix_leq_iter_tot(int vno);                /* ITER_TOT(VNO), VNO in 1..10 (leqsol_mod.f:17) */
+void mfix_leq_reset_iter_tot(void);
+const char *mfix_leq_error_string(int code);
+
+/* ---- drop-in solver entry (host arrays, blocking) ------------------------ */
+/* LEQ_BICGS model/leq_bicgs.f:23-90.  VAR/A_M/B_M are host arrays in the
+ * storage contract above; on return VAR holds the solution and A_M/B_M the
+ * row-scaled system, as the reference leaves them (leq_bicgs.f:194-233). */
+int mfix_leq_bicgs(const char *vname, int vname_len, int vno, double *var,
+                   double *a_m, double *b_m, const char *cmethod,
+                   int cmethod_len, double tol, const char *pc, int itmax,
+                   int *ier);
+
+/* SOLVE_LIN_EQ model/solve_lin_eq.f:17-195 for METHOD=2: A_M is
+ * DIMENSION_3 x 7 x (DIMENSION_M+1) and B_M DIMENSION_3 x (DIMENSION_M+1),
+ * phase M (0-based as in the reference) is sliced out (solve_lin_eq.f:151).
+ * Other METHODs return MFIX_LEQ_EMETHOD. */
+int mfix_solve_lin_eq(const char *vname, int vname_len, int vno, double *var,
+                      double *a_m, double *b_m, int m, int itmax, int method,
+                      const char *sweep, double tol1, const char *pc, int *ier);
+
+/* Options of the device-resident entry points. */
+typedef struct mfix_leq_opts {
+    int write_back_scaled;       /* host entry: copy scaled A_M/B_M back (1 = reference behaviour) */
+    const double *noise;         /* optional U[0,1) array (DIMENSION_3) for Rtilde (leq_bicgs.f:298-305); device ptr for _dev calls, host ptr for host calls; NULL = internal generator */
+    uint64_t noise_seed;         /* seed of the internal counter-based U[0,1) generator */
+    double inner_tol;            /* mixed precision: relative tolerance of each FP32 inner solve (0 = default 1e-5) */
+    int max_refine;              /* mixed precision: max FP64 refinement cycles (0 = default 40) */
+} mfix_leq_opts;
+
+/* Device-resident FP64 LEQ_BICGS: var/a_m/b_m are device pointers (storage
+ * contract above), stream may be 0.  Same semantics as mfix_leq_bicgs. */
+int mfix_leq_bicgs_dev(const char *cmethod, double tol, const char *pc,
+                       int itmax, double *d_var, double *d_a_m, double *d_b_m,
+                       const mfix_leq_opts *opts, mfix_leq_stats *stats,
+                       void *stream);
+
+/* Mixed precision: FP32 inner BiCGSTAB (matvec, preconditioner, Krylov
+ * vectors) inside FP64 iterative refinement; FP64 true residual, solution
+ * accumulation and scalar reductions.  Converged when
+ * ||b - A x|| <= tol * ||b - A x0|| in FP64 on the row-scaled system.
+ * No counterpart in the reference. */
+int mfix_leq_bicgs_mixed_dev(const char *cmethod, double tol, const char *pc,
+                             int itmax, double *d_var, double *d_a_m,
+                             double *d_b_m, const mfix_leq_opts *opts,
+                             mfix_leq_stats *stats, void *stream);
+
+/* ---- kernels of the path, device pointers (for parity tests) ------------ */
+/* LEQ_MATVEC leqsol_mod.f:103-265: y = A x over every cell. */
+int mfix_leq_matvec_dev(const double *d_a_m, const double *d_x, double *d_y,
+                        void *stream);
+/* LEQ_MSOLVE / MSOLVE1 / MSOLVE0: z = M^-1 r for pc LINE (with cmethod)/DIAG/NONE. */
+int mfix_leq_msolve_dev(const char *pc, const char *cmethod, const double *d_a_m,
+                        const double *d_r, double *d_z, void *stream);
+/* dot_product_par leqsol_mod.f:1094-1201 (grid's is_serial decides the domain). */
+int mfix_leq_dot_dev(const double *d_x, const double *d_y, double *result,
+                     void *stream);
+/* Row max-norm unit scaling, leq_bicgs.f:194-233; *ier = -2 on a zero row. */
+int mfix_leq_scale_dev(double *d_a_m, double *d_b_m, int *ier, void *stream);
+/* FP32 variants used by the mixed-precision inner solver. */
+int mfix_leq_matvec32_dev(const float *d_a_m, const float *d_x, float *d_y,
+                          void *stream);
+
+/* ---- synthetic systems (host only, deterministic, no GPU needed) -------- */
+/* kind 0: symmetric pressure-correction, faces 10^U[-1,1], A0=-sum-1e-3U, B=U[-1,1]
+ * kind 1: nonsymmetric first-order-upwind momentum, D=0.5*4^U, divergence-free
+ *         8-mode flux with max cell Peclet 10, transient rho V/dt at CFL 0.5
+ * kind 2: kind 0 faces times face-averaged eps_g (2000 spheres at 0.4 in 1.0),
+ *         B=U[-1e-3,1e-3]
+ * a_m: 7*N3 doubles, b_m: N3 doubles (storage contract above).  Ghost rows:
+ * A0=-1, off-diagonals 0, B=0. */
+int mfix_leq_gen_system(int kind, int imax, int jmax, int kmax, uint64_t seed,
+                        int nspheres, double *a_m, double *b_m);
+/* U[0,1) noise array from the same counter-based generator. */
+int mfix_leq_gen_uniform(int64_t n, uint64_t seed, uint64_t stream, double *out);
+
+/* ---- batched solves (many independent systems, one launch sequence) ---- */
+typedef struct mfix_leq_batch_item {
+    int imax, jmax, kmax;        /* grid of this system (serial layout, 3-D) */
+    char pc[4];                  /* "DIAG", "LINE" or "NONE" */
+    char cmethod[4];             /* LINE sweep string, e.g. "RSRS" */
+    double tol;
+    int itmax;
+    int mixed;                   /* 1 = FP32 inner + FP64 refinement */
+    double *d_var, *d_a_m, *d_b_m; /* device arrays of this system */
+    uint64_t noise_seed;
+} mfix_leq_batch_item;
+
+/* Solve every item; systems advance together, one fused launch per phase
+ * covering all systems (captured in a HIP graph).  stats may be NULL. */
+int mfix_leq_batch_solve(int nsys, const mfix_leq_batch_item *items,
+                         mfix_leq_stats *stats, void *stream);
+
+/* Autotuner: pick the block shape of the fused kernels for one grid size;
+ * results are cached per (n3 class, precision).  Returns the chosen
+ * threads-per-block and lines-per-block. */
+int mfix_leq_autotune(int imax, int jmax, int kmax, int mixed, int *block,
+                      int *lines_per_block);
+
+/* ---- serving: continuous batching with HBM-budget admission ------------- */
+typedef struct mfix_leq_server mfix_leq_server;
+typedef struct mfix_leq_request {
+    int imax, jmax, kmax;
+    char pc[4];
+    char cmethod[4];
+    double tol;
+    int itmax;
+    int mixed;
+    const double *a_m, *b_m;     /* host arrays (copied at admission) */
+    double *var;                 /* host in/out: initial guess, then solution */
+    uint64_t noise_seed;
+} mfix_leq_request;
+
+int mfix_leq_server_create(size_t hbm_budget_bytes, int max_slots,
+                           mfix_leq_server **out);
+/* Non-blocking: queue a request; returns its id (>=0) or a negative code. */
+int64_t mfix_leq_server_submit(mfix_leq_server *s, const mfix_leq_request *r);
+/* Advance the device by one scheduling round (admit, iterate, retire). */
+int mfix_leq_server_step(mfix_leq_server *s);
+/* Query a request: returns 1 when done (stats filled, var written), 0 if pending. */
+int mfix_leq_server_poll(mfix_leq_server *s, int64_t id, mfix_leq_stats *stats);
+int mfix_leq_server_pending(mfix_leq_server *s);
+int mfix_leq_server_destroy(mfix_leq_server *s);
+
+/* ---- timing of the dominant kernel (HIP events on the solver stream) ---- */
+/* Enable per-launch event timing of kernel `which` (0 = fused stencil
+ * kernel, 1 = line preconditioner kernel); read back the launch count, the
+ * summed milliseconds and the algorithmic bytes those launches moved. */
+int mfix_leq_prof_enable(int which, int on);
+int mfix_leq_prof_read(int which, int64_t *launches, double *total_ms,
+                       double *alg_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MFIX_LEQ_H */

@@ -1,0 +1,55 @@
+"""Synthetic MFIX-layout systems (host side, deterministic per seed).
+
+Thin numpy wrappers over ``mfix_leq_gen_system`` / ``mfix_leq_gen_uniform``
+of libmfix_leq.so.  Arrays follow the reference's serial storage: A_M is
+DIMENSION_3 x 7 column-major (seven contiguous planes l=-3..3, plane l at
+offset (l+3)*N3; model/ambm_mod.f:27-28), J is the stride-1 axis
+(model/include/functions.inc:37-42).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+
+PRESSURE, MOMENTUM, VOID_PRESSURE = 0, 1, 2
+KINDS = {"pressure": PRESSURE, "momentum": MOMENTUM, "void_pressure": VOID_PRESSURE}
+
+
+def dims(imax: int, jmax: int, kmax: int, no_k: bool = False) -> tuple[int, int, int, int]:
+    ni, nj, nk = imax + 2, jmax + 2, (1 if no_k else kmax + 2)
+    return ni, nj, nk, ni * nj * nk
+
+
+def gen_system(kind, imax: int, jmax: int, kmax: int, seed: int, nspheres: int = -1):
+    """Return (A_M (7*N3,), B_M (N3,)) float64 for a 3-D grid."""
+    k = KINDS.get(kind, kind) if isinstance(kind, str) else int(kind)
+    n3 = dims(imax, jmax, kmax)[3]
+    a = np.empty(7 * n3, dtype=np.float64)
+    b = np.empty(n3, dtype=np.float64)
+    rc = _lib.load().mfix_leq_gen_system(k, imax, jmax, kmax, C.c_uint64(seed), nspheres,
+                                         a.ctypes.data_as(_lib.c_double_p),
+                                         b.ctypes.data_as(_lib.c_double_p))
+    _lib.check(rc, "mfix_leq_gen_system")
+    return a, b
+
+
+def gen_uniform(n: int, seed: int, stream: int = 0) -> np.ndarray:
+    out = np.empty(n, dtype=np.float64)
+    _lib.check(_lib.load().mfix_leq_gen_uniform(n, C.c_uint64(seed), C.c_uint64(stream),
+                                               out.ctypes.data_as(_lib.c_double_p)),
+               "mfix_leq_gen_uniform")
+    return out
+
+
+def config_system(cfg: int, sys_index: int = 0, n: int | None = None):
+    """Systems of the benchmark configurations (seed = 1000*cfg + sys_index).
+
+    cfg 0: 32^3 symmetric pressure-correction; cfg 1: 128^3 upwind momentum;
+    cfg 2: 256^3 void-fraction pressure-correction.  ``n`` overrides the side.
+    """
+    side = {0: 32, 1: 128, 2: 256}[cfg] if n is None else n
+    kind = {0: PRESSURE, 1: MOMENTUM, 2: VOID_PRESSURE}[cfg]
+    return gen_system(kind, side, side, side, 1000 * cfg + sys_index)

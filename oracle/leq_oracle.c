@@ -1,0 +1,542 @@
+/*
+ * leq_oracle.c - CPU restatement of MFIX's LEQ_BICGS path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the parity checker for the HIP
+ * solver: only tests, the smoke check and the CPU-baseline leg of bench.py
+ * load it.  The product library never links or calls it.
+ *
+ * It follows the reference's serial semantics (numPEs=1, RE_INDEXING off):
+ *   layout/increments   model/include/functions.inc:37-42,
+ *                       model/dmp_modules/gridmap_mod.f:259-351,511-556,
+ *                       model/set_increments.f:72-112,173-178
+ *   unit scaling        model/leq_bicgs.f:194-233
+ *   LEQ_BICGS0          model/leq_bicgs.f:107-509
+ *   LEQ_MATVEC          model/leqsol_mod.f:103-265 (summation order :195-204)
+ *   LEQ_MSOLVE (LINE)   model/leqsol_mod.f:286-598, sweeps :767-1086
+ *   LEQ_MSOLVE0 (NONE)  model/leqsol_mod.f:616-663
+ *   LEQ_MSOLVE1 (DIAG)  model/leqsol_mod.f:681-749
+ *   DGTSV (NRHS=1)      model/DGTSV.f:107-163,233-246
+ *   dot_product_par(2)  model/leqsol_mod.f:1094-1295
+ * Pinning (see tests/test_oracle_*.py): golden vectors written by the
+ * compiled reference (tests/golden/), scipy's LAPACK dgtsv, brute-force
+ * numpy stencils, a sparse direct solve, and closed-form special cases.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdio.h>
+
+typedef struct {
+    int imax, jmax, kmax, no_k;
+    int cyclic_x, cyclic_y, cyclic_z;
+    int is_serial, minimize_dotproducts, icheck_bicgs;
+} og_grid;
+
+typedef struct {
+    int ni, nj, nk;
+    int64_t c1, c2, n3;
+    int do_k;
+    const og_grid *g;
+} og_dims;
+
+static og_dims dims_of(const og_grid *g) {
+    og_dims d;
+    d.g = g;
+    d.ni = g->imax + 2;
+    d.nj = g->jmax + 2;
+    d.nk = g->no_k ? 1 : g->kmax + 2;
+    d.do_k = !g->no_k;
+    d.c1 = d.nj;
+    d.c2 = (int64_t)d.nj * d.ni;
+    d.n3 = d.c2 * d.nk;
+    return d;
+}
+
+/* 0-based storage index of 1-based (i,j,k): functions.inc:41 with
+ * c0 = -c1 - c2 in the serial layout (gridmap_mod.f:553-556). */
+static inline int64_t IJK(const og_dims *d, int i, int j, int k) {
+    return (int64_t)(j - 1) + (int64_t)(i - 1) * d->c1 + (int64_t)(k - 1) * d->c2;
+}
+
+/* Neighbour index along one axis: set_increments.f:72-112.  n = number of
+ * cells on the axis including the two ghosts (IMIN3=1 .. IMAX3=n). */
+static inline int nb_minus(int i, int n, int cyclic) {
+    int shift = !(i == 1 || i == n);
+    if (cyclic && shift) {
+        /* IMAP_C(IMAP_C(i)-1): i-1 == 1 (ghost IMIN2) maps to IMAX1 = n-1 */
+        int t = i - 1;
+        return (t == 1) ? n - 1 : t;
+    }
+    return i - 1 < 1 ? 1 : i - 1;
+}
+static inline int nb_plus(int i, int n, int cyclic) {
+    int shift = !(i == 1 || i == n);
+    if (cyclic && shift) {
+        int t = i + 1;
+        return (t == n) ? 2 : t;
+    }
+    return i + 1 > n ? n : i + 1;
+}
+
+/* INCREMENT_FOR_MP(1..6) targets of cell (i,j,k): IM, IP, JM, JP, KM, KP. */
+static void neighbours(const og_dims *d, int i, int j, int k, int64_t nb[6]) {
+    const og_grid *g = d->g;
+    nb[0] = IJK(d, nb_minus(i, d->ni, g->cyclic_x), j, k);
+    nb[1] = IJK(d, nb_plus(i, d->ni, g->cyclic_x), j, k);
+    nb[2] = IJK(d, i, nb_minus(j, d->nj, g->cyclic_y), k);
+    nb[3] = IJK(d, i, nb_plus(j, d->nj, g->cyclic_y), k);
+    if (d->do_k) {
+        nb[4] = IJK(d, i, j, nb_minus(k, d->nk, g->cyclic_z));
+        nb[5] = IJK(d, i, j, nb_plus(k, d->nk, g->cyclic_z));
+    } else {
+        nb[4] = nb[5] = IJK(d, i, j, k);
+    }
+}
+
+#define A(l, c) a[((int64_t)((l) + 3)) * n3 + (c)]
+
+/* LEQ_MATVEC leqsol_mod.f:103-265, non-RE_INDEXING branch.  In the serial
+ * layout istart..iend etc. cover every cell (gridmap_mod.f:511-532).  The
+ * core-cell loop (:186-209) uses the first core cell's class, which equals
+ * each core cell's own increments, so one per-cell loop reproduces both. */
+void oracle_matvec(const og_grid *g, const double *a, const double *x, double *y) {
+    og_dims d = dims_of(g);
+    const int64_t n3 = d.n3;
+    for (int k = 1; k <= d.nk; ++k)
+        for (int i = 1; i <= d.ni; ++i)
+            for (int j = 1; j <= d.nj; ++j) {
+                int64_t c = IJK(&d, i, j, k), nb[6];
+                neighbours(&d, i, j, k, nb);
+                double v = A(-2, c) * x[nb[2]] + A(-1, c) * x[nb[0]] + A(0, c) * x[c]
+                         + A(1, c) * x[nb[1]] + A(2, c) * x[nb[3]];
+                if (d.do_k) {
+                    v = v + A(-3, c) * x[nb[4]];
+                    v = v + A(3, c) * x[nb[5]];
+                }
+                y[c] = v;
+            }
+}
+
+/* DGTSV with NRHS=1, DGTSV.f:107-163 (elimination with partial pivoting)
+ * and :233-246 (back solve).  dl has n-1 entries, d and du n, b n.
+ * Returns INFO (0 or the 1-based index of a zero pivot). */
+int oracle_dgtsv(int n, double *dl, double *d, double *du, double *b) {
+    if (n == 0) return 0;
+    for (int i = 0; i < n - 2; ++i) {
+        if (fabs(d[i]) >= fabs(dl[i])) {
+            if (d[i] != 0.0) {
+                double fact = dl[i] / d[i];
+                d[i + 1] = d[i + 1] - fact * du[i];
+                b[i + 1] = b[i + 1] - fact * b[i];
+            } else {
+                return i + 1;
+            }
+            dl[i] = 0.0;
+        } else {
+            double fact = d[i] / dl[i];
+            d[i] = dl[i];
+            double temp = d[i + 1];
+            d[i + 1] = du[i] - fact * temp;
+            dl[i] = du[i + 1];
+            du[i + 1] = -fact * dl[i];
+            du[i] = temp;
+            temp = b[i];
+            b[i] = b[i + 1];
+            b[i + 1] = temp - fact * b[i + 1];
+        }
+    }
+    if (n > 1) {
+        int i = n - 2;
+        if (fabs(d[i]) >= fabs(dl[i])) {
+            if (d[i] != 0.0) {
+                double fact = dl[i] / d[i];
+                d[i + 1] = d[i + 1] - fact * du[i];
+                b[i + 1] = b[i + 1] - fact * b[i];
+            } else {
+                return i + 1;
+            }
+        } else {
+            double fact = d[i] / dl[i];
+            d[i] = dl[i];
+            double temp = d[i + 1];
+            d[i + 1] = du[i] - fact * temp;
+            du[i] = temp;
+            temp = b[i];
+            b[i] = b[i + 1];
+            b[i + 1] = temp - fact * b[i + 1];
+        }
+    }
+    if (d[n - 1] == 0.0) return n;
+    b[n - 1] = b[n - 1] / d[n - 1];
+    if (n > 1) b[n - 2] = (b[n - 2] - du[n - 2] * b[n - 1]) / d[n - 2];
+    for (int i = n - 3; i >= 0; --i)
+        b[i] = (b[i] - du[i] * b[i + 1] - dl[i] * b[i + 2]) / d[i];
+    return 0;
+}
+
+/* Line along J at (i,k): LEQ_IKSWEEP leqsol_mod.f:851-921 (3-D) and
+ * LEQ_ISWEEP :767-834 (2-D, k=1, no K terms).  On DGTSV failure the line
+ * keeps its current values (:910-914).  Returns INFO. */
+static int sweep_j_line(const og_dims *d, int i, int k, const double *a, const double *bm, double *var,
+                        double *cc, double *dd, double *ee, double *bb) {
+    const int64_t n3 = d->n3;
+    const int n = d->nj;
+    for (int j = 1; j <= n; ++j) {
+        int64_t c = IJK(d, i, j, k), nb[6];
+        neighbours(d, i, j, k, nb);
+        dd[j - 1] = A(0, c);
+        cc[j - 1] = A(-2, c);
+        ee[j - 1] = A(2, c);
+        if (d->do_k)
+            bb[j - 1] = bm[c] - A(-1, c) * var[nb[0]] - A(1, c) * var[nb[1]] - A(-3, c) * var[nb[4]]
+                      - A(3, c) * var[nb[5]];
+        else
+            bb[j - 1] = bm[c] - A(-1, c) * var[nb[0]] - A(1, c) * var[nb[1]];
+    }
+    cc[0] = 0.0;
+    ee[n - 1] = 0.0;
+    int info = oracle_dgtsv(n, cc + 1, dd, ee, bb);
+    if (info != 0) return info;
+    for (int j = 1; j <= n; ++j) var[IJK(d, i, j, k)] = bb[j - 1];
+    return 0;
+}
+
+/* Line along I at (j,k): LEQ_JKSWEEP leqsol_mod.f:939-1004 (failure is
+ * fatal in the reference: returns INFO to the caller). */
+static int sweep_i_line(const og_dims *d, int j, int k, const double *a, const double *bm, double *var,
+                        double *cc, double *dd, double *ee, double *bb) {
+    const int64_t n3 = d->n3;
+    const int n = d->ni;
+    for (int i = 1; i <= n; ++i) {
+        int64_t c = IJK(d, i, j, k), nb[6];
+        neighbours(d, i, j, k, nb);
+        dd[i - 1] = A(0, c);
+        cc[i - 1] = A(-1, c);
+        ee[i - 1] = A(1, c);
+        bb[i - 1] = bm[c] - A(-2, c) * var[nb[2]] - A(2, c) * var[nb[3]] - A(-3, c) * var[nb[4]]
+                  - A(3, c) * var[nb[5]];
+    }
+    cc[0] = 0.0;
+    ee[n - 1] = 0.0;
+    int info = oracle_dgtsv(n, cc + 1, dd, ee, bb);
+    if (info != 0) return info;
+    for (int i = 1; i <= n; ++i) var[IJK(d, i, j, k)] = bb[i - 1];
+    return 0;
+}
+
+/* Line along K at (i,j): LEQ_IJSWEEP leqsol_mod.f:1021-1086 (fatal on failure). */
+static int sweep_k_line(const og_dims *d, int i, int j, const double *a, const double *bm, double *var,
+                        double *cc, double *dd, double *ee, double *bb) {
+    const int64_t n3 = d->n3;
+    const int n = d->nk;
+    for (int k = 1; k <= n; ++k) {
+        int64_t c = IJK(d, i, j, k), nb[6];
+        neighbours(d, i, j, k, nb);
+        dd[k - 1] = A(0, c);
+        cc[k - 1] = A(-3, c);
+        ee[k - 1] = A(3, c);
+        bb[k - 1] = bm[c] - A(-2, c) * var[nb[2]] - A(2, c) * var[nb[3]] - A(-1, c) * var[nb[0]]
+                  - A(1, c) * var[nb[1]];
+    }
+    cc[0] = 0.0;
+    ee[n - 1] = 0.0;
+    int info = oracle_dgtsv(n, cc + 1, dd, ee, bb);
+    if (info != 0) return info;
+    for (int k = 1; k <= n; ++k) var[IJK(d, i, j, k)] = bb[k - 1];
+    return 0;
+}
+
+/* Position p (1-based) of the reference's red-black enumeration of a
+ * 2-D index set of size s1 (fast) x s2 (slow), leqsol_mod.f:404-418:
+ * returns the (fast, slow) 1-based offsets added to the range starts. */
+static void rb_position(int p, int s1, int lo2, int *f, int *s) {
+    int slow;
+    if (p % s1 != 0) slow = p / s1 + lo2;
+    else slow = p / s1 + lo2 - 1;
+    int fast = (p - 1 - (slow - lo2) * s1) + 1 + (slow % 2); /* + j1 with j1 = 1 */
+    if (fast > s1) fast = fast - s1 + 1 - 1;                 /* j - j2 + j1 - 1 with j1=1, j2=s1 */
+    *f = fast;
+    *s = slow;
+}
+
+/* LEQ_MSOLVE leqsol_mod.f:286-598.  var = b (SETGUESS, :341-357), then one
+ * pass per character of the 4-character CMETHOD.  Returns 0, or a positive
+ * DGTSV INFO from a JK/IJ sweep, which the reference treats as fatal. */
+int oracle_msolve_line(const og_grid *g, const double *bm, const double *a, double *var, const char *cmethod) {
+    og_dims d = dims_of(g);
+    int nmax = d.ni > d.nj ? d.ni : d.nj;
+    if (d.nk > nmax) nmax = d.nk;
+    double *cc = malloc(sizeof(double) * nmax), *dd = malloc(sizeof(double) * nmax);
+    double *ee = malloc(sizeof(double) * nmax), *bb = malloc(sizeof(double) * nmax);
+    int fatal = 0;
+    memcpy(var, bm, sizeof(double) * d.n3);
+    for (int it = 0; it < 4 && !fatal; ++it) {
+        char ch = cmethod[it];
+        int do_i = (ch == 'I' || ch == 'i'), do_k = (ch == 'K' || ch == 'k');
+        int do_all = (ch == 'A' || ch == 'a'), do_rb = (ch == 'R' || ch == 'r');
+        if (!d.do_k) {
+            /* two-dimensional (:367-392) */
+            if (do_i)
+                for (int i = 1; i <= d.ni; ++i) sweep_j_line(&d, i, 1, a, bm, var, cc, dd, ee, bb);
+            if (do_rb) {
+                for (int i = 1; i <= d.ni; i += 2) sweep_j_line(&d, i, 1, a, bm, var, cc, dd, ee, bb);
+                for (int i = 2; i <= d.ni; i += 2) sweep_j_line(&d, i, 1, a, bm, var, cc, dd, ee, bb);
+            }
+            continue;
+        }
+        if (do_all) {
+            /* JK lines (along I), :399-421 */
+            int jsize = d.nj, ksize = d.nk;
+            for (int icase = 1; icase <= 2 && !fatal; ++icase)
+                for (int p = icase; p <= ksize * jsize; p += 2) {
+                    int j, k;
+                    rb_position(p, jsize, 1, &j, &k);
+                    int info = sweep_i_line(&d, j, k, a, bm, var, cc, dd, ee, bb);
+                    if (info) { fatal = info; break; }
+                }
+            /* IJ lines (along K), :425-446 */
+            int isize = d.ni;
+            jsize = d.nj;
+            for (int icase = 1; icase <= 2 && !fatal; ++icase)
+                for (int p = icase; p <= jsize * isize; p += 2) {
+                    int i, j;
+                    rb_position(p, isize, 1, &i, &j);
+                    int info = sweep_k_line(&d, i, j, a, bm, var, cc, dd, ee, bb);
+                    if (info) { fatal = info; break; }
+                }
+            /* IK lines (along J), :450-469 */
+            isize = d.ni;
+            ksize = d.nk;
+            for (int icase = 1; icase <= 2 && !fatal; ++icase)
+                for (int p = icase; p <= ksize * isize; p += 2) {
+                    int i, k;
+                    rb_position(p, isize, 1, &i, &k);
+                    sweep_j_line(&d, i, k, a, bm, var, cc, dd, ee, bb);
+                }
+        }
+        if (do_rb && !fatal) {
+            /* :497-522: k odd -> i = istart+1, +3..; k even -> i = istart, +2.. */
+            for (int k = 1; k <= d.nk; ++k)
+                for (int i = (k % 2 != 0) ? 2 : 1; i <= d.ni; i += 2) sweep_j_line(&d, i, k, a, bm, var, cc, dd, ee, bb);
+            for (int k = 1; k <= d.nk; ++k)
+                for (int i = (k % 2 != 0) ? 1 : 2; i <= d.ni; i += 2) sweep_j_line(&d, i, k, a, bm, var, cc, dd, ee, bb);
+        }
+        if (!fatal) {
+            /* USE_IKLOOP = .FALSE. branch, :570-585 */
+            if (do_i)
+                for (int k = 1; k <= d.nk; ++k)
+                    for (int i = 1; i <= d.ni; ++i) sweep_j_line(&d, i, k, a, bm, var, cc, dd, ee, bb);
+            if (do_k)
+                for (int i = 1; i <= d.ni; ++i)
+                    for (int k = 1; k <= d.nk; ++k) sweep_j_line(&d, i, k, a, bm, var, cc, dd, ee, bb);
+        }
+    }
+    free(cc); free(dd); free(ee); free(bb);
+    return fatal;
+}
+
+/* LEQ_MSOLVE1 leqsol_mod.f:681-749: var = 0, then b/A0 over *start2..*end2
+ * (every cell in the serial layout). */
+void oracle_msolve_diag(const og_grid *g, const double *bm, const double *a, double *var) {
+    og_dims d = dims_of(g);
+    const int64_t n3 = d.n3;
+    for (int64_t c = 0; c < n3; ++c) var[c] = bm[c] / A(0, c);
+}
+
+/* LEQ_MSOLVE0 leqsol_mod.f:616-663. */
+void oracle_msolve_none(const og_grid *g, const double *bm, double *var) {
+    og_dims d = dims_of(g);
+    memcpy(var, bm, sizeof(double) * d.n3);
+}
+
+/* dot_product_par leqsol_mod.f:1094-1201.  Serial default: the intrinsic
+ * over all DIMENSION_3 cells in storage order (:1125-1136).  Otherwise the
+ * interior cells *start1..*end1 in (k,i,j) loop order (:1152-1162). */
+double oracle_dot(const og_grid *g, const double *x, const double *y) {
+    og_dims d = dims_of(g);
+    double s = 0.0;
+    if (g->is_serial) {
+        for (int64_t c = 0; c < d.n3; ++c) s += x[c] * y[c];
+        return s;
+    }
+    int k0 = d.do_k ? 2 : 1, k1 = d.do_k ? d.nk - 1 : 1;
+    for (int k = k0; k <= k1; ++k)
+        for (int i = 2; i <= d.ni - 1; ++i)
+            for (int j = 2; j <= d.nj - 1; ++j) {
+                int64_t c = IJK(&d, i, j, k);
+                s += x[c] * y[c];
+            }
+    return s;
+}
+
+/* dot_product_par2 leqsol_mod.f:1210-1295 (interior cells, no dead cells
+ * without a cut-cell grid). */
+void oracle_dot2(const og_grid *g, const double *r1, const double *r2, const double *r3, const double *r4,
+                 double out[2]) {
+    og_dims d = dims_of(g);
+    double p0 = 0.0, p1 = 0.0;
+    int k0 = d.do_k ? 2 : 1, k1 = d.do_k ? d.nk - 1 : 1;
+    for (int k = k0; k <= k1; ++k)
+        for (int i = 2; i <= d.ni - 1; ++i)
+            for (int j = 2; j <= d.nj - 1; ++j) {
+                int64_t c = IJK(&d, i, j, k);
+                p0 += r1[c] * r2[c];
+                p1 += r3[c] * r4[c];
+            }
+    out[0] = p0;
+    out[1] = p1;
+}
+
+/* Row max-norm unit scaling, leq_bicgs.f:194-233 (loop over *start2..*end2,
+ * every cell in serial).  Returns 0 or -2 on a zero row. */
+int oracle_unit_scale(const og_grid *g, double *a, double *bm) {
+    og_dims d = dims_of(g);
+    const int64_t n3 = d.n3;
+    int ier = 0;
+    for (int64_t c = 0; c < n3; ++c) {
+        double amax = 0.0;
+        for (int l = -3; l <= 3; ++l) {
+            double v = fabs(A(l, c));
+            if (v > amax) amax = v;
+        }
+        if (amax > 0.0) {
+            double oam = 1.0 / amax;
+            for (int l = -3; l <= 3; ++l) A(l, c) = A(l, c) * oam;
+            bm[c] = bm[c] * oam;
+        } else {
+            ier = -2;
+        }
+    }
+    return ier;
+}
+
+typedef struct {
+    int iterations, iter_tot_increment, ier, exit_reason;
+    double rnorm0, rnorm;
+    int msolve_fatal;
+} og_result;
+
+/* LEQ_BICGS (PC dispatch, leq_bicgs.f:74-87) + LEQ_BICGS0 (:107-509).
+ * noise: the U[0,1) array random_number would fill Rtilde with (:298).
+ * rnorm_trace (optional, itmax+1 entries) receives Rnorm0 then each Rnorm.
+ * Returns 0, or -1 for an unknown PC (the reference aborts). */
+int oracle_leq_bicgs(const og_grid *g, const char *pc, const char *cmethod, double tol, int itmax,
+                     double *var, double *a, double *bm, const double *noise, og_result *res,
+                     double *rnorm_trace) {
+    int mode;
+    if (strncmp(pc, "LINE", 4) == 0) mode = 2;
+    else if (strncmp(pc, "DIAG", 4) == 0) mode = 1;
+    else if (strncmp(pc, "NONE", 4) == 0) mode = 0;
+    else return -1;
+    const int use_pc = mode != 0;
+    og_dims d = dims_of(g);
+    const int64_t n = d.n3;
+    memset(res, 0, sizeof(*res));
+    res->ier = oracle_unit_scale(g, a, bm);
+    if (res->ier != 0) { res->exit_reason = 4; return 0; }
+
+    double *R = calloc(n, 8), *Rt = calloc(n, 8), *P = calloc(n, 8), *Pp = calloc(n, 8);
+    double *S = calloc(n, 8), *Sp = calloc(n, 8), *T = calloc(n, 8), *V = calloc(n, 8);
+    double *alpha = calloc(itmax + 2, 8), *omega = calloc(itmax + 2, 8), *rho = calloc(itmax + 2, 8);
+    const double tolmin = 2.220446049250313e-16; /* EPSILON(1d0) */
+    const double ratiotol = (double)0.2f;          /* DOUBLE PRECISION :: ratiotol = 0.2 (default-real literal) */
+    double rnorm = 0.0, rnorm0, snorm;
+    int isconverged = 0, iter = 1, done = 0;
+
+    oracle_matvec(g, a, var, R);
+    for (int64_t c = 0; c < n; ++c) R[c] = bm[c] - R[c];
+    rnorm0 = sqrt(oracle_dot(g, R, R));
+    for (int64_t c = 0; c < n; ++c) Rt[c] = R[c] + (2.0 * noise[c] - 1.0) * 1.0e-6 * rnorm0;
+    if (rnorm_trace) rnorm_trace[0] = rnorm0;
+
+    double *Phat = P, *Shat = S;
+    int i;
+    for (i = 1; i <= itmax; ++i) {
+        rho[i - 1] = oracle_dot(g, Rt, R);
+        if (rho[i - 1] == 0.0) {
+            res->ier = (i != 1) ? -2 : 0;
+            res->exit_reason = 2;
+            res->iterations = i - 1;
+            done = 1;
+            break;
+        }
+        if (i == 1) {
+            memcpy(P, R, sizeof(double) * n);
+        } else {
+            double beta = (rho[i - 1] / rho[i - 2]) * (alpha[i - 1] / omega[i - 1]);
+            for (int64_t c = 0; c < n; ++c) P[c] = R[c] + beta * (P[c] - omega[i - 1] * V[c]);
+        }
+        if (use_pc) {
+            if (mode == 2) res->msolve_fatal |= oracle_msolve_line(g, P, a, Pp, cmethod);
+            else oracle_msolve_diag(g, P, a, Pp);
+            Phat = Pp;
+        } else {
+            Phat = P;
+        }
+        oracle_matvec(g, a, Phat, V);
+        double rtv = oracle_dot(g, Rt, V);
+        alpha[i] = rho[i - 1] / rtv;
+        for (int64_t c = 0; c < n; ++c) S[c] = R[c] - alpha[i] * V[c];
+        if (!g->minimize_dotproducts) {
+            snorm = sqrt(oracle_dot(g, S, S));
+            if (snorm <= tolmin) {
+                for (int64_t c = 0; c < n; ++c) var[c] = var[c] + alpha[i] * Phat[c];
+                isconverged = 1;
+                res->exit_reason = 1;
+                res->iterations = i;
+                break;
+            }
+        }
+        if (use_pc) {
+            if (mode == 2) res->msolve_fatal |= oracle_msolve_line(g, S, a, Sp, cmethod);
+            else oracle_msolve_diag(g, S, a, Sp);
+            Shat = Sp;
+        } else {
+            Shat = S;
+        }
+        oracle_matvec(g, a, Shat, T);
+        double txs, txt;
+        if (!g->minimize_dotproducts) {
+            txs = oracle_dot(g, T, S);
+            txt = oracle_dot(g, T, T);
+        } else {
+            double two[2];
+            oracle_dot2(g, T, S, T, T, two);
+            txs = two[0];
+            txt = two[1];
+        }
+        if (txt == 0.0) txt = 1.0e-15; /* SMALL_NUMBER param1_mod.f:24 */
+        omega[i] = txs / txt;
+        for (int64_t c = 0; c < n; ++c) var[c] = var[c] + alpha[i] * Phat[c] + omega[i] * Shat[c];
+        for (int64_t c = 0; c < n; ++c) R[c] = S[c] - omega[i] * T[c];
+        int icheck = g->icheck_bicgs > 0 ? g->icheck_bicgs : 1;
+        if (!g->minimize_dotproducts || (iter % icheck) == 0) {
+            rnorm = sqrt(oracle_dot(g, R, R));
+            if (rnorm_trace) rnorm_trace[i] = rnorm;
+            isconverged = (rnorm <= tol * rnorm0);
+            if (isconverged) {
+                res->iter_tot_increment = iter + 1;
+                res->exit_reason = 0;
+                res->iterations = i;
+                break;
+            }
+        }
+        iter = iter + 1;
+    }
+    if (!done) {
+        if (i > itmax) { res->iterations = itmax; res->exit_reason = 3; }
+        if (!isconverged) isconverged = ((double)(float)rnorm <= tol * rnorm0);
+        res->ier = 0;
+        if (!isconverged) {
+            res->ier = -1;
+            res->iter_tot_increment = iter;
+            if ((double)(float)rnorm >= ratiotol * (double)(float)rnorm0) res->ier = -2;
+        }
+    }
+    res->rnorm0 = rnorm0;
+    res->rnorm = rnorm;
+    free(R); free(Rt); free(P); free(Pp); free(S); free(Sp); free(T); free(V);
+    free(alpha); free(omega); free(rho);
+    return 0;
+}

@@ -1,0 +1,239 @@
+! ref_harness.f90 - driver around the reference's own LEQ_BICGS path.
+!
+! TEST INFRASTRUCTURE ONLY.  Built by oracle/Makefile (target `ref`) against
+! the reference modules compiled from model/ of the MFIX tree; the binary lives
+! in oracle/_ref/ and writes golden vectors for tests/golden/.  It sets the
+! serial module state that GRIDMAP_INIT / SET_INCREMENTS would set
+! (model/dmp_modules/gridmap_mod.f:259-351,511-556; model/set_increments.f:
+! 72-178; model/set_max2.f:53-69) and calls:
+!   op 1  LEQ_BICGS        model/leq_bicgs.f:23
+!   op 2  LEQ_MATVEC       model/leqsol_mod.f:103
+!   op 3  LEQ_MSOLVE       model/leqsol_mod.f:286  (LINE, CMETHOD)
+!   op 4  LEQ_MSOLVE1      model/leqsol_mod.f:681  (DIAG)
+!   op 5  dot_product_par  model/leqsol_mod.f:1094
+!   op 6  dot_product_par2 model/leqsol_mod.f:1210
+! Input file (stream): 12 int32 (op, imax, jmax, kmax, no_k, cyc_x, cyc_y,
+! cyc_z, is_serial, minimize_dotproducts, icheck_bicgs, itmax), int32 seed,
+! real64 tol, 4+4 chars pc/cmethod, then A(N3*7), B(N3), X(N3) real64.
+! Output file (stream): int32 ier, int32 iter_tot, real64 dot results(2),
+! X(N3), A(N3*7), B(N3), noise(N3).
+program ref_harness
+  use param, only: dimension_3
+  use compar
+  use geometry
+  use indices
+  use leqsol
+  use parallel, only: is_serial, use_doloop
+  use cutcell, only: re_indexing, cartesian_grid
+  use funits, only: dmp_log
+  implicit none
+  integer :: hdr(12), seed, op, itmax
+  double precision :: tol
+  character(len=4) :: pc, cmethod
+  character(len=512) :: fin, fout
+  double precision, allocatable :: amat(:,:), bvec(:), xvec(:), yvec(:), noise(:)
+  integer, allocatable :: seedarr(:), saved(:)
+  integer :: nseed, ier, ni, nj, nk, i, j, k, ijk, n3, ncls
+  integer :: im, ip, jm, jp, km, kp, key, cls
+  integer, allocatable :: classof(:)
+  double precision :: dres(2)
+
+  call get_command_argument(1, fin)
+  call get_command_argument(2, fout)
+  open(10, file=trim(fin), access='stream', form='unformatted', status='old')
+  read(10) hdr
+  read(10) seed
+  read(10) tol
+  read(10) pc, cmethod
+  op = hdr(1); itmax = hdr(12)
+
+  ! ---- serial grid state (set_max2.f:53-69, gridmap_mod.f) ----
+  imax = hdr(2); jmax = hdr(3); kmax = hdr(4)
+  no_k = hdr(5) /= 0
+  do_k = .not. no_k
+  no_i = .false.; no_j = .false.; do_i = .true.; do_j = .true.
+  cyclic_x = hdr(6) /= 0; cyclic_y = hdr(7) /= 0; cyclic_z = hdr(8) /= 0
+  ni = imax + 2; nj = jmax + 2
+  if (no_k) then
+     nk = 1
+  else
+     nk = kmax + 2
+  end if
+  myPE = 0; numPEs = 1; nodesi = 1; nodesj = 1; nodesk = 1; root = 0
+  dmp_log = .false.
+  re_indexing = .false.; cartesian_grid = .false.
+  is_serial = hdr(9) /= 0
+  use_doloop = .false.
+  minimize_dotproducts = hdr(10) /= 0
+  icheck_bicgs = max(1, hdr(11))
+  nlayers_bicgs = 1
+  imin1 = 2; imax1 = imax + 1; imin2 = 1; imax2 = imax + 2; imin3 = 1; imax3 = imax + 2
+  jmin1 = 2; jmax1 = jmax + 1; jmin2 = 1; jmax2 = jmax + 2; jmin3 = 1; jmax3 = jmax + 2
+  if (no_k) then
+     kmin1 = 1; kmax1 = 1; kmin2 = 1; kmax2 = 1; kmin3 = 1; kmax3 = 1
+  else
+     kmin1 = 2; kmax1 = kmax + 1; kmin2 = 1; kmax2 = kmax + 2; kmin3 = 1; kmax3 = kmax + 2
+  end if
+  istart1 = imin1; iend1 = imax1; istart2 = imin2; iend2 = imax2; istart3 = imin3; iend3 = imax3
+  jstart1 = jmin1; jend1 = jmax1; jstart2 = jmin2; jend2 = jmax2; jstart3 = jmin3; jend3 = jmax3
+  kstart1 = kmin1; kend1 = kmax1; kstart2 = kmin2; kend2 = kmax2; kstart3 = kmin3; kend3 = kmax3
+  istart = istart2; iend = iend2; jstart = jstart2; jend = jend2; kstart = kstart2; kend = kend2
+  c1 = nj
+  c2 = nj * ni
+  c0 = 1 - jstart3 - c1 * istart3 - c2 * kstart3
+  n3 = ni * nj * nk
+  dimension_3 = n3
+  ijkstart3 = 1; ijkend3 = n3; ijksize3 = n3
+  allocate(ijk_array_of(istart3:iend3, jstart3:jend3, kstart3:kend3))
+  allocate(funijk_map_c(istart3:iend3, jstart3:jend3, kstart3:kend3))
+  allocate(imap_c(imin3:imax3), jmap_c(jmin3:jmax3), kmap_c(kmin3:kmax3))
+  do i = imin3, imax3; imap_c(i) = i; end do
+  do j = jmin3, jmax3; jmap_c(j) = j; end do
+  do k = kmin3, kmax3; kmap_c(k) = k; end do
+  if (cyclic_x) then; imap_c(imax2) = imin1; imap_c(imin2) = imax1; end if
+  if (cyclic_y) then; jmap_c(jmax2) = jmin1; jmap_c(jmin2) = jmax1; end if
+  if (cyclic_z .and. do_k) then; kmap_c(kmax2) = kmin1; kmap_c(kmin2) = kmax1; end if
+  do k = kstart3, kend3
+     do i = istart3, iend3
+        do j = jstart3, jend3
+           ijk_array_of(i, j, k) = j + c0 + i * c1 + k * c2
+        end do
+     end do
+  end do
+  do k = kstart3, kend3
+     do i = istart3, iend3
+        do j = jstart3, jend3
+           funijk_map_c(i, j, k) = ijk_array_of(imap_c(i), jmap_c(j), kmap_c(k))
+        end do
+     end do
+  end do
+
+  ! ---- neighbour classes (set_increments.f:72-112,173-178) ----
+  if (allocated(cell_class)) deallocate(cell_class)
+  allocate(cell_class(n3))
+  allocate(classof(0:124))
+  classof = 0
+  ncls = 0
+  do k = kstart3, kend3
+     do i = istart3, iend3
+        do j = jstart3, jend3
+           ijk = ijk_array_of(i, j, k)
+           im = nbm(i, imin3, imax3, cyclic_x, imap_c)
+           ip = nbp(i, imin3, imax3, cyclic_x, imap_c)
+           jm = nbm(j, jmin3, jmax3, cyclic_y, jmap_c)
+           jp = nbp(j, jmin3, jmax3, cyclic_y, jmap_c)
+           if (do_k) then
+              km = nbm(k, kmin3, kmax3, cyclic_z, kmap_c)
+              kp = nbp(k, kmin3, kmax3, cyclic_z, kmap_c)
+           else
+              km = k; kp = k
+           end if
+           key = edge(i, imin3, imax3) + 5 * edge(j, jmin3, jmax3) + 25 * edge(k, kmin3, kmax3)
+           if (classof(key) == 0) then
+              ncls = ncls + 1
+              classof(key) = ncls
+              cls = ncls
+              increment_for_mp(1, cls) = ijk_array_of(im, j, k) - ijk
+              increment_for_mp(2, cls) = ijk_array_of(ip, j, k) - ijk
+              increment_for_mp(3, cls) = ijk_array_of(i, jm, k) - ijk
+              increment_for_mp(4, cls) = ijk_array_of(i, jp, k) - ijk
+              increment_for_mp(5, cls) = ijk_array_of(i, j, km) - ijk
+              increment_for_mp(6, cls) = ijk_array_of(i, j, kp) - ijk
+           end if
+           cell_class(ijk) = classof(key)
+        end do
+     end do
+  end do
+  use_corecell_loop = (ni >= 5 .and. nj >= 5 .and. (nk >= 5 .or. no_k))
+
+  allocate(amat(n3, -3:3), bvec(n3), xvec(n3), yvec(n3), noise(n3))
+  read(10) amat
+  read(10) bvec
+  read(10) xvec
+  close(10)
+
+  iter_tot = 0
+  ier = 0
+  dres = 0d0
+  noise = 0d0
+  select case (op)
+  case (1)
+     call random_seed(size=nseed)
+     allocate(seedarr(nseed), saved(nseed))
+     do i = 1, nseed
+        seedarr(i) = seed + 7919 * i
+     end do
+     call random_seed(put=seedarr)
+     call random_seed(get=saved)
+     call leq_bicgs('HARNESS', 1, xvec, amat, bvec, cmethod, tol, pc, itmax, ier)
+     call random_seed(put=saved)
+     call random_number(noise)
+     yvec = xvec
+  case (2)
+     call leq_matvec('HARNESS', xvec, amat, yvec)
+  case (3)
+     call leq_msolve('HARNESS', xvec, amat, yvec, cmethod)
+  case (4)
+     call leq_msolve1('HARNESS', xvec, amat, yvec, cmethod)
+  case (5)
+     dres(1) = dot_product_par(xvec, bvec)
+     yvec = xvec
+  case (6)
+     dres = dot_product_par2(xvec, bvec, xvec, xvec)
+     yvec = xvec
+  end select
+
+  open(11, file=trim(fout), access='stream', form='unformatted', status='replace')
+  write(11) ier, iter_tot(1)
+  write(11) dres
+  write(11) yvec
+  write(11) amat
+  write(11) bvec
+  write(11) noise
+  close(11)
+
+contains
+
+  integer function edge(i, lo, hi)
+    integer, intent(in) :: i, lo, hi
+    if (i == lo) then
+       edge = 1
+    else if (i == hi) then
+       edge = 2
+    else if (i == lo + 1) then
+       edge = 3
+    else if (i == hi - 1) then
+       edge = 4
+    else
+       edge = 0
+    end if
+  end function edge
+
+  ! IM1/IP1 of set_increments.f:72-112 for one axis (NODES=1)
+  integer function nbm(i, lo, hi, cyc, map)
+    integer, intent(in) :: i, lo, hi
+    logical, intent(in) :: cyc
+    integer, intent(in) :: map(lo:hi)
+    logical :: shift
+    shift = .not. (i == lo .or. i == hi)
+    if (cyc .and. shift) then
+       nbm = map(map(i) - 1)
+    else
+       nbm = max(lo, i - 1)
+    end if
+  end function nbm
+
+  integer function nbp(i, lo, hi, cyc, map)
+    integer, intent(in) :: i, lo, hi
+    logical, intent(in) :: cyc
+    integer, intent(in) :: map(lo:hi)
+    logical :: shift
+    shift = .not. (i == lo .or. i == hi)
+    if (cyc .and. shift) then
+       nbp = map(map(i) + 1)
+    else
+       nbp = min(hi, i + 1)
+    end if
+  end function nbp
+
+end program ref_harness
