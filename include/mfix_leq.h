@@ -71,6 +71,7 @@ typedef struct mfix_leq_stats {
     int refinements;             /* FP64 refinement cycles (mixed precision), 0 for FP64 solves */
     int inner_iterations;        /* FP32 inner iterations summed over cycles (mixed) */
     double device_ms;            /* device time of the solve measured with HIP events */
+    int restarts;                /* mixed precision: recurrence restarts after a drifting update */
 } mfix_leq_stats;
 
 /* ---- lifetime ----------------------------------------------------------- */
@@ -201,6 +202,27 @@ int mfix_leq_server_step(mfix_leq_server *s);
 int mfix_leq_server_poll(mfix_leq_server *s, int64_t id, mfix_leq_stats *stats);
 int mfix_leq_server_pending(mfix_leq_server *s);
 int mfix_leq_server_destroy(mfix_leq_server *s);
+
+/* ---- device memory helpers (resident A_M/B_M/VAR for the _dev entries) -- */
+int mfix_leq_dev_alloc(size_t bytes, void **out);
+int mfix_leq_dev_free(void *p);
+int mfix_leq_copy_to_dev(void *dst, const void *src, size_t bytes);
+int mfix_leq_copy_from_dev(void *dst, const void *src, size_t bytes);
+int mfix_leq_copy_dev(void *dst, const void *src, size_t bytes);
+/* Plain streaming copy kernel of n doubles (HBM traffic calibration:
+ * reads 8n and writes 8n bytes). */
+int mfix_leq_copy_kernel_dev(double *dst, const double *src, int64_t n);
+int mfix_leq_sync(void);
+
+/* ---- parity / diagnostics hooks of the host entry ------------------------ */
+/* Use this U[0,1) array (host, DIMENSION_3) for Rtilde in mfix_leq_bicgs
+ * instead of the internal generator; NULL restores the generator (seed). */
+int mfix_leq_set_noise(const double *host_noise, uint64_t seed);
+/* 1 (default): mfix_leq_bicgs copies the row-scaled A_M/B_M back as the
+ * reference leaves them; 0: only VAR comes back (no caller reads them). */
+int mfix_leq_set_write_back(int on);
+/* Statistics of the last mfix_leq_bicgs call. */
+int mfix_leq_last_stats(mfix_leq_stats *st);
 
 /* ---- timing of the dominant kernel (HIP events on the solver stream) ---- */
 /* Enable per-launch event timing of kernel `which` (0 = fused stencil

@@ -29,7 +29,7 @@ class Stats(C.Structure):
         ("iterations", C.c_int), ("iter_tot_increment", C.c_int), ("ier", C.c_int),
         ("exit_reason", C.c_int), ("rnorm0", C.c_double), ("rnorm", C.c_double),
         ("true_rnorm", C.c_double), ("refinements", C.c_int), ("inner_iterations", C.c_int),
-        ("device_ms", C.c_double),
+        ("device_ms", C.c_double), ("restarts", C.c_int),
     ]
 
     def as_dict(self) -> dict:
@@ -100,6 +100,16 @@ _SIGS = {
     "mfix_leq_server_poll": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(Stats)]),
     "mfix_leq_server_pending": (C.c_int, [C.c_void_p]),
     "mfix_leq_server_destroy": (C.c_int, [C.c_void_p]),
+    "mfix_leq_dev_alloc": (C.c_int, [C.c_size_t, C.POINTER(C.c_void_p)]),
+    "mfix_leq_dev_free": (C.c_int, [C.c_void_p]),
+    "mfix_leq_copy_to_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "mfix_leq_copy_from_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "mfix_leq_copy_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "mfix_leq_sync": (C.c_int, []),
+    "mfix_leq_copy_kernel_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
+    "mfix_leq_set_noise": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "mfix_leq_set_write_back": (C.c_int, [C.c_int]),
+    "mfix_leq_last_stats": (C.c_int, [C.POINTER(Stats)]),
     "mfix_leq_prof_enable": (C.c_int, [C.c_int, C.c_int]),
     "mfix_leq_prof_read": (C.c_int, [C.c_int, C.POINTER(C.c_int64), c_double_p, c_double_p]),
 }

@@ -1,0 +1,361 @@
+// C ABI of the gfx950 LEQ_BICGS solver (include/mfix_leq.h).  Every entry
+// validates its arguments, selects the context's stream and calls the solver
+// engine (leq_solve.hip); none of them computes on the host.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+
+#include "../../include/mfix_leq.h"
+#include "leq_context.h"
+
+using namespace mfix;
+
+namespace {
+
+const char *err_name(int code) {
+    switch (code) {
+    case MFIX_LEQ_OK: return "ok";
+    case MFIX_LEQ_EBADPC: return "preconditioner option not found (PC must be LINE, DIAG or NONE)";
+    case MFIX_LEQ_ENOGRID: return "grid not initialised (call mfix_leq_grid_init)";
+    case MFIX_LEQ_EHIP: return "HIP runtime error";
+    case MFIX_LEQ_EINVAL: return "invalid argument";
+    case MFIX_LEQ_ENOMEM: return "device memory allocation failed or over budget";
+    case MFIX_LEQ_EDGTSV: return "DGTSV failed in a JK/IJ line sweep";
+    case MFIX_LEQ_EBUSY: return "serving queue full";
+    case MFIX_LEQ_EMETHOD: return "only LEQ_METHOD=2 (BiCGSTAB) is provided";
+    default: return "unknown error";
+    }
+}
+
+int ready(hipStream_t *s, void *stream) {
+    Context &C = ctx();
+    int rc;
+    if ((rc = C.ensure_init())) return rc;
+    if (!C.have_grid) return MFIX_LEQ_ENOGRID;
+    *s = stream ? (hipStream_t)stream : C.stream;
+    return 0;
+}
+
+int sync_status(hipStream_t s, const char *what) {
+    hipError_t e = hipStreamSynchronize(s);
+    if (e == hipSuccess) e = hipGetLastError();
+    return e == hipSuccess ? 0 : hip_fail(e, what);
+}
+
+int copy_sync(void *dst, const void *src, size_t bytes, hipMemcpyKind kind) {
+    int rc;
+    if ((rc = ctx().ensure_init())) return rc;
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, ctx().stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx().stream);
+    return e == hipSuccess ? 0 : hip_fail(e, "hipMemcpy");
+}
+
+}  // namespace
+
+extern "C" {
+
+int mfix_leq_init(int device) { return ctx().init(device); }
+
+int mfix_leq_finalize(void) {
+    Context &C = ctx();
+    if (!C.initialised) return 0;
+    (void)hipStreamSynchronize(C.stream);
+    C.release_all();
+    if (C.host_state) (void)hipHostFree(C.host_state);
+    C.host_state = nullptr;
+    (void)hipStreamDestroy(C.stream);
+    C.stream = nullptr;
+    C.initialised = false;
+    return 0;
+}
+
+int mfix_leq_grid_init(const mfix_leq_grid *g) {
+    if (!g) return MFIX_LEQ_EINVAL;
+    if (g->imax < 1 || g->jmax < 1 || (!g->no_k && g->kmax < 1)) return MFIX_LEQ_ENOGRID;
+    const double n3 = (double)(g->imax + 2) * (g->jmax + 2) * (g->no_k ? 1 : g->kmax + 2);
+    if (n3 >= 2147483647.0) return MFIX_LEQ_EINVAL;  // kernels index cells with 32-bit (i,j,k) math
+    Context &C = ctx();
+    C.grid = *g;
+    if (C.grid.icheck_bicgs < 1) C.grid.icheck_bicgs = 1;
+    C.have_grid = true;
+    return 0;
+}
+
+int64_t mfix_leq_dimension_3(void) {
+    Context &C = ctx();
+    if (!C.have_grid) return MFIX_LEQ_ENOGRID;
+    return make_dgrid(C.grid).n3;
+}
+
+int mfix_leq_iter_tot(int vno) {
+    if (vno < 1 || vno > 10) return MFIX_LEQ_EINVAL;
+    return ctx().iter_tot[vno - 1];
+}
+
+void mfix_leq_reset_iter_tot(void) { std::memset(ctx().iter_tot, 0, sizeof(ctx().iter_tot)); }
+
+const char *mfix_leq_error_string(int code) {
+    if (code == MFIX_LEQ_EHIP && ctx().last_msg[0]) return ctx().last_msg;
+    return err_name(code);
+}
+
+int mfix_leq_bicgs_dev(const char *cmethod, double tol, const char *pc, int itmax, double *d_var, double *d_a_m,
+                       double *d_b_m, const mfix_leq_opts *opts, mfix_leq_stats *stats, void *stream) {
+    hipStream_t s;
+    int rc;
+    if ((rc = ready(&s, stream))) return rc;
+    const int p = parse_pc(pc);
+    if (p < 0) return MFIX_LEQ_EBADPC;
+    if (!d_var || !d_a_m || !d_b_m) return MFIX_LEQ_EINVAL;
+    const Context &C = ctx();
+    return solve_fp64(make_dgrid(C.grid), p, cmethod ? cmethod : "RSRS", tol, itmax, d_var, d_a_m, d_b_m, opts, stats,
+                      s, C.grid.minimize_dotproducts, C.grid.icheck_bicgs);
+}
+
+int mfix_leq_bicgs_mixed_dev(const char *cmethod, double tol, const char *pc, int itmax, double *d_var,
+                             double *d_a_m, double *d_b_m, const mfix_leq_opts *opts, mfix_leq_stats *stats,
+                             void *stream) {
+    hipStream_t s;
+    int rc;
+    if ((rc = ready(&s, stream))) return rc;
+    const int p = parse_pc(pc);
+    if (p < 0) return MFIX_LEQ_EBADPC;
+    if (!d_var || !d_a_m || !d_b_m) return MFIX_LEQ_EINVAL;
+    return solve_mixed(make_dgrid(ctx().grid), p, cmethod ? cmethod : "RSRS", tol, itmax, d_var, d_a_m, d_b_m, opts,
+                       stats, s);
+}
+
+// Host-array drop-in for LEQ_BICGS (model/leq_bicgs.f:23-90).
+int mfix_leq_bicgs(const char *vname, int vname_len, int vno, double *var, double *a_m, double *b_m,
+                   const char *cmethod, int cmethod_len, double tol, const char *pc, int itmax, int *ier) {
+    (void)vname;
+    (void)vname_len;
+    Context &C = ctx();
+    hipStream_t s;
+    int rc;
+    if ((rc = ready(&s, nullptr))) return rc;
+    const int p = parse_pc(pc);
+    if (p < 0) return MFIX_LEQ_EBADPC;
+    if (!var || !a_m || !b_m || !ier) return MFIX_LEQ_EINVAL;
+    char cm[5] = "    ";
+    for (int q = 0; q < 4; ++q) cm[q] = (cmethod && q < cmethod_len) ? cmethod[q] : ' ';
+    const DGrid g = make_dgrid(C.grid);
+    const int64_t n3 = g.n3;
+    if ((rc = C.d_a.ensure(sizeof(double) * 7 * n3)) || (rc = C.d_b.ensure(sizeof(double) * n3)) ||
+        (rc = C.d_x.ensure(sizeof(double) * n3)))
+        return rc;
+    double *da = (double *)C.d_a.ptr, *db = (double *)C.d_b.ptr, *dx = (double *)C.d_x.ptr;
+    hipError_t e;
+    if ((e = hipMemcpyAsync(da, a_m, sizeof(double) * 7 * n3, hipMemcpyHostToDevice, s)) != hipSuccess ||
+        (e = hipMemcpyAsync(db, b_m, sizeof(double) * n3, hipMemcpyHostToDevice, s)) != hipSuccess ||
+        (e = hipMemcpyAsync(dx, var, sizeof(double) * n3, hipMemcpyHostToDevice, s)) != hipSuccess)
+        return hip_fail(e, "host to device copy");
+    mfix_leq_opts o = {};
+    o.write_back_scaled = 1;
+    if (C.host_noise) {
+        if ((rc = C.misc.ensure(sizeof(double) * n3))) return rc;
+        if ((e = hipMemcpyAsync(C.misc.ptr, C.host_noise, sizeof(double) * n3, hipMemcpyHostToDevice, s)) !=
+            hipSuccess)
+            return hip_fail(e, "noise copy");
+        o.noise = (const double *)C.misc.ptr;
+    }
+    o.noise_seed = C.noise_seed;
+    mfix_leq_stats st = {};
+    rc = solve_fp64(g, p, cm, tol, itmax, dx, da, db, &o, &st, s, C.grid.minimize_dotproducts, C.grid.icheck_bicgs);
+    if (rc && rc != MFIX_LEQ_EDGTSV) return rc;
+    if ((e = hipMemcpyAsync(var, dx, sizeof(double) * n3, hipMemcpyDeviceToHost, s)) != hipSuccess)
+        return hip_fail(e, "device to host copy");
+    if (C.write_back_scaled) {
+        if ((e = hipMemcpyAsync(a_m, da, sizeof(double) * 7 * n3, hipMemcpyDeviceToHost, s)) != hipSuccess ||
+            (e = hipMemcpyAsync(b_m, db, sizeof(double) * n3, hipMemcpyDeviceToHost, s)) != hipSuccess)
+            return hip_fail(e, "device to host copy");
+    }
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    *ier = st.ier;
+    if (vno >= 1 && vno <= 10) C.iter_tot[vno - 1] += st.iter_tot_increment;
+    C.last_stats = st;
+    return rc;
+}
+
+// SOLVE_LIN_EQ (model/solve_lin_eq.f:17-195), METHOD = 2 only.
+int mfix_solve_lin_eq(const char *vname, int vname_len, int vno, double *var, double *a_m, double *b_m, int m,
+                      int itmax, int method, const char *sweep, double tol1, const char *pc, int *ier) {
+    if (method != 2) return MFIX_LEQ_EMETHOD;
+    Context &C = ctx();
+    if (!C.have_grid) return MFIX_LEQ_ENOGRID;
+    if (m < 0) return MFIX_LEQ_EINVAL;
+    const int64_t n3 = make_dgrid(C.grid).n3;
+    // A_M(:,:,M) and B_M(:,M) are contiguous slices (M is the last dimension)
+    return mfix_leq_bicgs(vname, vname_len, vno, var, a_m + (int64_t)m * 7 * n3, b_m + (int64_t)m * n3, sweep, 4,
+                          tol1, pc, itmax, ier);
+}
+
+int mfix_leq_matvec_dev(const double *d_a_m, const double *d_x, double *d_y, void *stream) {
+    hipStream_t s;
+    int rc;
+    if ((rc = ready(&s, stream))) return rc;
+    if (!d_a_m || !d_x || !d_y) return MFIX_LEQ_EINVAL;
+    run_matvec64(make_dgrid(ctx().grid), d_a_m, d_x, d_y, s);
+    return sync_status(s, "matvec");
+}
+
+int mfix_leq_matvec32_dev(const float *d_a_m, const float *d_x, float *d_y, void *stream) {
+    hipStream_t s;
+    int rc;
+    if ((rc = ready(&s, stream))) return rc;
+    if (!d_a_m || !d_x || !d_y) return MFIX_LEQ_EINVAL;
+    run_matvec32(make_dgrid(ctx().grid), d_a_m, d_x, d_y, s);
+    return sync_status(s, "matvec32");
+}
+
+int mfix_leq_msolve_dev(const char *pc, const char *cmethod, const double *d_a_m, const double *d_r, double *d_z,
+                        void *stream) {
+    hipStream_t s;
+    int rc;
+    if ((rc = ready(&s, stream))) return rc;
+    const int p = parse_pc(pc);
+    if (p < 0) return MFIX_LEQ_EBADPC;
+    if (!d_a_m || !d_r || !d_z) return MFIX_LEQ_EINVAL;
+    Context &C = ctx();
+    const DGrid g = make_dgrid(C.grid);
+    hipError_t e;
+    if (p == PC_NONE) {
+        e = hipMemcpyAsync(d_z, d_r, sizeof(double) * g.n3, hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) return hip_fail(e, "copy");
+        return sync_status(s, "msolve");
+    }
+    if (p == PC_DIAG) {
+        run_diag(g, d_a_m, d_r, d_z, s);
+        return sync_status(s, "msolve");
+    }
+    if ((rc = C.line_scratch.ensure(sizeof(double) * g.n3 * 4))) return rc;
+    if ((rc = C.misc.ensure(sizeof(int) * 4 + sizeof(double) * 8))) return rc;
+    int *flag = (int *)C.misc.ptr;
+    e = hipMemsetAsync(flag, 0, sizeof(int), s);
+    if (e != hipSuccess) return hip_fail(e, "memset");
+    launch_msolve_line<double>(g, C.shape_for(g.n3, false), cmethod ? cmethod : "RSRS", d_a_m, d_r, d_z,
+                               (double *)C.line_scratch.ptr, nullptr, 0, flag, s);
+    int hflag = 0;
+    e = hipMemcpyAsync(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return hip_fail(e, "copy");
+    if ((rc = sync_status(s, "msolve"))) return rc;
+    for (auto &pp : C.prof) pp.collect();
+    return hflag ? MFIX_LEQ_EDGTSV : 0;
+}
+
+int mfix_leq_dot_dev(const double *d_x, const double *d_y, double *result, void *stream) {
+    hipStream_t s;
+    int rc;
+    if ((rc = ready(&s, stream))) return rc;
+    if (!result || !d_x || !d_y) return MFIX_LEQ_EINVAL;
+    Context &C = ctx();
+    if ((rc = C.misc.ensure(sizeof(double) * 8))) return rc;
+    double *out = (double *)C.misc.ptr;
+    if ((rc = run_dot(make_dgrid(C.grid), d_x, d_y, out, s))) return rc;
+    hipError_t e = hipMemcpyAsync(result, out, sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return hip_fail(e, "copy");
+    return sync_status(s, "dot");
+}
+
+int mfix_leq_scale_dev(double *d_a_m, double *d_b_m, int *ier, void *stream) {
+    hipStream_t s;
+    int rc;
+    if ((rc = ready(&s, stream))) return rc;
+    if (!ier || !d_a_m || !d_b_m) return MFIX_LEQ_EINVAL;
+    int zero = 0;
+    if ((rc = run_scale(make_dgrid(ctx().grid), d_a_m, d_b_m, &zero, s))) return rc;
+    *ier = zero ? -2 : 0;
+    return 0;
+}
+
+int mfix_leq_autotune(int imax, int jmax, int kmax, int mixed, int *block, int *lines_per_block) {
+    if (imax < 1 || jmax < 1 || kmax < 1) return MFIX_LEQ_EINVAL;
+    Shape best;
+    int rc = autotune(imax, jmax, kmax, mixed, &best);
+    if (rc) return rc;
+    if (block) *block = best.bs * 100 + best.cpt;
+    if (lines_per_block) *lines_per_block = best.lines_per_block;
+    return 0;
+}
+
+int mfix_leq_prof_enable(int which, int on) {
+    if (which < 0 || which >= kNumProf) return MFIX_LEQ_EINVAL;
+    Prof &p = ctx().prof[which];
+    p.on = on != 0;
+    p.launches = 0;
+    p.total_ms = 0.0;
+    p.alg_bytes = 0.0;
+    return 0;
+}
+
+int mfix_leq_prof_read(int which, int64_t *launches, double *total_ms, double *alg_bytes) {
+    if (which < 0 || which >= kNumProf) return MFIX_LEQ_EINVAL;
+    const Prof &p = ctx().prof[which];
+    if (launches) *launches = p.launches;
+    if (total_ms) *total_ms = p.total_ms;
+    if (alg_bytes) *alg_bytes = p.alg_bytes;
+    return 0;
+}
+
+int mfix_leq_set_noise(const double *host_noise, uint64_t seed) {
+    ctx().host_noise = host_noise;
+    ctx().noise_seed = seed;
+    return 0;
+}
+
+int mfix_leq_set_write_back(int on) {
+    ctx().write_back_scaled = on != 0;
+    return 0;
+}
+
+int mfix_leq_last_stats(mfix_leq_stats *st) {
+    if (!st) return MFIX_LEQ_EINVAL;
+    *st = ctx().last_stats;
+    return 0;
+}
+
+int mfix_leq_dev_alloc(size_t bytes, void **out) {
+    int rc;
+    if ((rc = ctx().ensure_init())) return rc;
+    if (!out) return MFIX_LEQ_EINVAL;
+    *out = nullptr;
+    hipError_t e = hipMalloc(out, bytes ? bytes : 1);
+    if (e != hipSuccess) {
+        hip_fail(e, "hipMalloc");
+        return MFIX_LEQ_ENOMEM;
+    }
+    return 0;
+}
+
+int mfix_leq_dev_free(void *p) {
+    if (!p) return 0;
+    hipError_t e = hipFree(p);
+    return e == hipSuccess ? 0 : hip_fail(e, "hipFree");
+}
+
+int mfix_leq_copy_to_dev(void *dst, const void *src, size_t bytes) {
+    return copy_sync(dst, src, bytes, hipMemcpyHostToDevice);
+}
+int mfix_leq_copy_from_dev(void *dst, const void *src, size_t bytes) {
+    return copy_sync(dst, src, bytes, hipMemcpyDeviceToHost);
+}
+int mfix_leq_copy_dev(void *dst, const void *src, size_t bytes) {
+    return copy_sync(dst, src, bytes, hipMemcpyDeviceToDevice);
+}
+
+int mfix_leq_copy_kernel_dev(double *dst, const double *src, int64_t n) {
+    int rc;
+    if ((rc = ctx().ensure_init())) return rc;
+    if (!dst || !src || n < 0) return MFIX_LEQ_EINVAL;
+    run_copy(src, dst, n, ctx().stream);
+    return sync_status(ctx().stream, "copy kernel");
+}
+
+int mfix_leq_sync(void) {
+    int rc;
+    if ((rc = ctx().ensure_init())) return rc;
+    hipError_t e = hipDeviceSynchronize();
+    return e == hipSuccess ? 0 : hip_fail(e, "hipDeviceSynchronize");
+}
+
+}  // extern "C"

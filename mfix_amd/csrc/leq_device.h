@@ -1,0 +1,305 @@
+// Device-side building blocks of the gfx950 LEQ_BICGS solver.
+//
+// Storage follows the reference's serial layout (include/mfix_leq.h): J is
+// the stride-1 axis, seven SoA coefficient planes, one ghost layer.  Plane
+// index p = l + 3 for stencil slot l (model/param_mod.f:28-50):
+//   p0 = bottom K-, p1 = south J-, p2 = west I-, p3 = diagonal,
+//   p4 = east I+, p5 = north J+, p6 = top K+.
+//
+// Floating-point contraction is off in every translation unit that includes
+// this header, so FP64 products and sums round exactly as the reference's
+// x86-64 build does (separate multiply and add, left-to-right order of
+// model/leqsol_mod.f:195-204).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#pragma clang fp contract(off)
+
+namespace mfix {
+
+constexpr int kNumXcd = 8;       // MI355X: 8 XCDs, blocks dispatched round-robin
+constexpr int kFinishBlock = 1024;
+
+// Grid descriptor passed by value to every kernel.
+struct DGrid {
+    int ni, nj, nk;       // cells per axis including the two ghosts (nk=1 in 2-D)
+    int do_k;             // 3-D run
+    int cyc_x, cyc_y, cyc_z;
+    int dot_all;          // dot_product_par domain: 1 = every cell (IS_SERIAL), 0 = interior
+    int64_t c1, c2, n3;   // strides of I and K, DIMENSION_3
+};
+
+// Neighbour along one axis (0-based), set_increments.f:72-112: clamped to the
+// cell itself at the array ends; wrapped for cyclic axes on interior cells.
+__device__ __forceinline__ int nb_minus(int i, int n, int cyc) {
+    const bool shift = !(i == 0 || i == n - 1);
+    if (cyc && shift) { const int t = i - 1; return t == 0 ? n - 2 : t; }
+    return i > 0 ? i - 1 : 0;
+}
+__device__ __forceinline__ int nb_plus(int i, int n, int cyc) {
+    const bool shift = !(i == 0 || i == n - 1);
+    if (cyc && shift) { const int t = i + 1; return t == n - 1 ? 1 : t; }
+    return i < n - 1 ? i + 1 : n - 1;
+}
+
+struct Nb {
+    int64_t im, ip, jm, jp, km, kp;
+    bool interior;
+};
+
+// (i,j,k) of storage index c (0-based).  DIMENSION_3 < 2^31 (checked at grid
+// set-up), so 32-bit unsigned division suffices.
+__device__ __forceinline__ void cell_ijk(const DGrid &g, int64_t c, int &i, int &j, int &k) {
+    const uint32_t cc = (uint32_t)c, c2 = (uint32_t)g.c2, nj = (uint32_t)g.nj;
+    const uint32_t kk = cc / c2;
+    const uint32_t m = cc - kk * c2;
+    const uint32_t ii = m / nj;
+    i = (int)ii;
+    j = (int)(m - ii * nj);
+    k = (int)kk;
+}
+
+__device__ __forceinline__ bool is_interior(const DGrid &g, int64_t c) {
+    int i, j, k;
+    cell_ijk(g, c, i, j, k);
+    const bool ij = i > 0 && i < g.ni - 1 && j > 0 && j < g.nj - 1;
+    return g.do_k ? (ij && k > 0 && k < g.nk - 1) : ij;
+}
+
+__device__ __forceinline__ Nb neighbours(const DGrid &g, int64_t c) {
+    int i, j, k;
+    cell_ijk(g, c, i, j, k);
+    Nb n;
+    n.jm = c + (nb_minus(j, g.nj, g.cyc_y) - j);
+    n.jp = c + (nb_plus(j, g.nj, g.cyc_y) - j);
+    n.im = c + (int64_t)(nb_minus(i, g.ni, g.cyc_x) - i) * g.c1;
+    n.ip = c + (int64_t)(nb_plus(i, g.ni, g.cyc_x) - i) * g.c1;
+    if (g.do_k) {
+        n.km = c + (int64_t)(nb_minus(k, g.nk, g.cyc_z) - k) * g.c2;
+        n.kp = c + (int64_t)(nb_plus(k, g.nk, g.cyc_z) - k) * g.c2;
+        n.interior = i > 0 && i < g.ni - 1 && j > 0 && j < g.nj - 1 && k > 0 && k < g.nk - 1;
+    } else {
+        n.km = n.kp = c;
+        n.interior = i > 0 && i < g.ni - 1 && j > 0 && j < g.nj - 1;
+    }
+    return n;
+}
+
+// Neighbours with a fast path: away from the two outermost layers every
+// cell has the fixed increments -1/+1, -c1/+c1, -c2/+c2 (cyclic wraps and
+// clamps only touch layers 0, 1, n-2, n-1), so only a thin shell of cells
+// evaluates the per-axis rules.
+__device__ __forceinline__ Nb neighbours_fast(const DGrid &g, int64_t c) {
+    int i, j, k;
+    cell_ijk(g, c, i, j, k);
+    const bool core = i > 1 && i < g.ni - 2 && j > 1 && j < g.nj - 2 && (!g.do_k || (k > 1 && k < g.nk - 2));
+    Nb n;
+    if (core) {
+        n.jm = c - 1;
+        n.jp = c + 1;
+        n.im = c - g.c1;
+        n.ip = c + g.c1;
+        n.km = g.do_k ? c - g.c2 : c;
+        n.kp = g.do_k ? c + g.c2 : c;
+        n.interior = true;
+        return n;
+    }
+    n.jm = c + (nb_minus(j, g.nj, g.cyc_y) - j);
+    n.jp = c + (nb_plus(j, g.nj, g.cyc_y) - j);
+    n.im = c + (int64_t)(nb_minus(i, g.ni, g.cyc_x) - i) * g.c1;
+    n.ip = c + (int64_t)(nb_plus(i, g.ni, g.cyc_x) - i) * g.c1;
+    if (g.do_k) {
+        n.km = c + (int64_t)(nb_minus(k, g.nk, g.cyc_z) - k) * g.c2;
+        n.kp = c + (int64_t)(nb_plus(k, g.nk, g.cyc_z) - k) * g.c2;
+        n.interior = i > 0 && i < g.ni - 1 && j > 0 && j < g.nj - 1 && k > 0 && k < g.nk - 1;
+    } else {
+        n.km = n.kp = c;
+        n.interior = i > 0 && i < g.ni - 1 && j > 0 && j < g.nj - 1;
+    }
+    return n;
+}
+
+// Streaming loads of the coefficient planes (read once per sweep): the
+// non-temporal hint keeps them from displacing the reused vector in L2.
+template <typename T>
+__device__ __forceinline__ T ld_stream(const T *p) {
+    return __builtin_nontemporal_load(p);
+}
+
+// Operand fetch of the stencil: x itself, or x/A0 of the neighbour (DIAG
+// preconditioner applied on the fly, LEQ_MSOLVE1 leqsol_mod.f:729-744, so
+// Phat and Shat never exist as arrays).  Both forms round exactly like the
+// reference's separate division and matvec.
+template <typename T, bool DIV>
+struct Operand {
+    const T *__restrict__ x;
+    const T *__restrict__ a0;
+    __device__ __forceinline__ T operator()(int64_t q) const {
+        if (DIV) return (T)((double)x[q] / (double)a0[q]);
+        return x[q];
+    }
+};
+
+// y(c) of LEQ_MATVEC in the reference's summation order.
+template <typename T, typename X>
+__device__ __forceinline__ T stencil_row(const DGrid &g, const T *__restrict__ a, const X &x, int64_t c,
+                                         const Nb &n) {
+    const int64_t n3 = g.n3;
+    T v = ld_stream(&a[1 * n3 + c]) * x(n.jm) + ld_stream(&a[2 * n3 + c]) * x(n.im);
+    v = v + ld_stream(&a[3 * n3 + c]) * x(c);
+    v = v + ld_stream(&a[4 * n3 + c]) * x(n.ip);
+    v = v + ld_stream(&a[5 * n3 + c]) * x(n.jp);
+    if (g.do_k) {
+        v = v + ld_stream(&a[0 * n3 + c]) * x(n.km);
+        v = v + ld_stream(&a[6 * n3 + c]) * x(n.kp);
+    }
+    return v;
+}
+
+// FP32 stencil in difference form: with the diagonal plane replaced by the
+// row excess e = A0 + sum(off-diagonals) (formed in FP64, then rounded),
+// y = sum_l a_l (x_nb - x_c) + e x_c is algebraically LEQ_MATVEC but avoids
+// the cancellation between A0 x_c and the neighbour terms that costs a
+// diffusion operator's smooth modes most of FP32's digits.
+template <typename X>
+__device__ __forceinline__ float stencil_row_diff(const DGrid &g, const float *__restrict__ a, const X &x,
+                                                  int64_t c, const Nb &n) {
+    const int64_t n3 = g.n3;
+    const float xc = x(c);
+    double v = (double)(ld_stream(&a[1 * n3 + c]) * (x(n.jm) - xc)) +
+               (double)(ld_stream(&a[2 * n3 + c]) * (x(n.im) - xc));
+    v += (double)(ld_stream(&a[4 * n3 + c]) * (x(n.ip) - xc));
+    v += (double)(ld_stream(&a[5 * n3 + c]) * (x(n.jp) - xc));
+    if (g.do_k) {
+        v += (double)(ld_stream(&a[0 * n3 + c]) * (x(n.km) - xc));
+        v += (double)(ld_stream(&a[6 * n3 + c]) * (x(n.kp) - xc));
+    }
+    v += (double)ld_stream(&a[3 * n3 + c]) * (double)xc;
+    return (float)v;
+}
+
+// ---- launch shape ---------------------------------------------------------
+//
+// A launch has nblk logical blocks of BS threads; logical block b owns the
+// contiguous cells [b*chunk, (b+1)*chunk), chunk = BS * cells-per-thread.
+// Hardware places block b on XCD b % 8; the remap below gives XCD x one
+// contiguous range of logical blocks, i.e. one slab of the grid, so the
+// I/K neighbours its blocks read were brought into the same L2.
+struct Part {
+    int64_t nblk, chunk, n;
+};
+
+inline Part make_part(int64_t n, int bs, int cpt) {
+    Part p;
+    p.n = n;
+    p.chunk = (int64_t)bs * (cpt > 0 ? cpt : 1);
+    p.nblk = (n + p.chunk - 1) / p.chunk;
+    if (p.nblk < 1) p.nblk = 1;
+    return p;
+}
+
+__device__ __forceinline__ int64_t xcd_block(int64_t nblocks) {
+    const int64_t b = blockIdx.x;
+    const int64_t per = (nblocks + kNumXcd - 1) / kNumXcd;
+    const int64_t x = b % kNumXcd, r = b / kNumXcd;
+    const int64_t full = nblocks - (per - 1) * kNumXcd;  // XCDs that get `per` blocks
+    const int64_t start = x < full ? x * per : full * per + (x - full) * (per - 1);
+    return start + r;
+}
+// Hardware blocks to launch for nblocks logical ones (every XCD gets
+// ceil(nblocks/8) slots; surplus slots exit immediately).
+__host__ __device__ inline int64_t xcd_grid(int64_t nblocks) {
+    const int64_t per = (nblocks + kNumXcd - 1) / kNumXcd;
+    return per * kNumXcd;
+}
+__device__ __forceinline__ bool xcd_valid(int64_t nblocks) {
+    const int64_t b = blockIdx.x;
+    const int64_t per = (nblocks + kNumXcd - 1) / kNumXcd;
+    const int64_t x = b % kNumXcd, r = b / kNumXcd;
+    const int64_t full = nblocks - (per - 1) * kNumXcd;
+    return r < (x < full ? per : per - 1);
+}
+
+// ---- deterministic reductions (fixed tree, wave64 shuffles) ----------------
+// block_sum leaves the block totals in thread 0; lds holds NV * (BS/64).
+template <int BS, int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double *lds) {
+    constexpr int NW = BS / 64;
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        double s = v[q];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+        v[q] = s;
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) lds[q * NW + w] = v[q];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+            double s = lds[q * NW + 0];
+            for (int u = 1; u < NW; ++u) s += lds[q * NW + u];
+            v[q] = s;
+        }
+    }
+}
+
+// Grid-wide sums in two steps.  Each block stores its NV partial sums with
+// plain stores; a one-block finish kernel launched next sums them in
+// block-index order.  The kernel boundary makes the partials visible across
+// the eight XCDs' L2s once per launch, where a per-block release fence
+// (last-block-done pattern) would write back each XCD's L2 for every block.
+template <int BS, int NV>
+__device__ __forceinline__ void block_partial(double (&v)[NV], double *partials, int64_t nblk, int64_t blk,
+                                              double *lds) {
+    block_sum<BS, NV>(v, lds);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) partials[q * nblk + blk] = v[q];
+    }
+}
+
+// Inside a kFinishBlock-thread finish kernel: totals of the NV partial arrays
+// (valid in thread 0), summed in a fixed order.  Loads are issued in groups
+// of four so a thread's partials arrive together instead of one round trip
+// each.
+template <int NV>
+__device__ __forceinline__ void finish_sum(const double *partials, int64_t nblk, double *lds, double (&tot)[NV]) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        const double *p = partials + q * nblk;
+        double s = 0.0;
+        int64_t b = threadIdx.x;
+        for (; b + 3 * kFinishBlock < nblk; b += 4 * kFinishBlock) {
+            const double v0 = p[b], v1 = p[b + kFinishBlock], v2 = p[b + 2 * kFinishBlock],
+                         v3 = p[b + 3 * kFinishBlock];
+            s += v0;
+            s += v1;
+            s += v2;
+            s += v3;
+        }
+        for (; b < nblk; b += kFinishBlock) s += p[b];
+        tot[q] = s;
+    }
+    block_sum<kFinishBlock, NV>(tot, lds);
+}
+
+// Counter-based U[0,1) (same construction as the host generator).
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+__device__ __forceinline__ double urand(uint64_t seed, uint64_t stream, uint64_t idx) {
+    uint64_t h = mix64(seed * 0x9E3779B97F4A7C15ULL + 0x632BE59BD9B4E019ULL);
+    h = mix64(h ^ (stream * 0xD1B54A32D192ED03ULL));
+    h = mix64(h + idx * 0xAEF17502108EF2D9ULL);
+    return (double)(h >> 11) * (1.0 / 9007199254740992.0);
+}
+
+}  // namespace mfix

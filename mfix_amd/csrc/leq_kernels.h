@@ -1,0 +1,477 @@
+// Kernels of the BiCGSTAB path (LEQ_BICGS0, model/leq_bicgs.f:107-509).
+//
+// One solve keeps its Krylov scalars and control flags in a device-resident
+// SolverState.  Each reduction phase is a streaming kernel that writes one
+// partial sum per block, followed by a one-block finish kernel that sums the
+// partials in a fixed order and applies the reference's scalar logic (alpha,
+// the S-norm exit, omega, the convergence test, IER).  No iteration needs a
+// host round trip, a chunk of iterations can be captured in a HIP graph, and
+// every kernel returns at once when the solve is done, which makes extra
+// launches after convergence harmless.
+//
+// Every streaming kernel runs a fixed number of blocks; logical block b owns
+// the contiguous cells [b*chunk, (b+1)*chunk) and XCD x owns a contiguous
+// range of logical blocks, so the I/K neighbours a block reads were loaded
+// into the same XCD's L2 by its neighbouring blocks.
+//
+// Vector/matrix storage type T is double (plain FP64 solve) or float (inner
+// solve of the mixed-precision path); element-wise updates are evaluated in
+// FP64 registers and rounded once to T, reductions accumulate in FP64.
+// FP32 matrices use the difference layout (8 planes: six off-diagonals, the
+// row excess A0+sum(off) in plane 3, A0 in plane 7), see stencil_row_diff.
+#pragma once
+#include "leq_device.h"
+
+namespace mfix {
+
+enum Pc : int { PC_NONE = 0, PC_DIAG = 1, PC_LINE = 2 };
+
+struct SolverState {
+    double rnorm0, rnorm, snorm, rho, rho_prev, alpha, omega, beta;
+    double tol, rtv, txs, txt;
+    double true_rnorm;      // mixed: last FP64 ||b - A x||
+    double upd_rnorm;       // mixed: ||r|| at the last reliable update
+    double rnorm_max;       // mixed: max recursive ||r|| since the last update
+    double delta;           // mixed: reliable-update trigger factor
+    int i;                  // current loop index (1-based), leq_bicgs.f:316
+    int iter_var;           // the reference's `iter` counter
+    int itmax;
+    int done, ier, exit_reason, s_exit, iter_tot_inc, isconverged;
+    int minimize, icheck;
+    int zero_row;
+    int need_update;        // mixed: reliable update requested by the last iteration
+    int msolve_fatal;       // DGTSV INFO>0 in a JK/IJ line sweep (fatal in the reference)
+    int updates;            // mixed: reliable updates performed
+    int mixed;              // 1 = inner solve of the mixed-precision path
+    int final_update;       // mixed: the pending update ends the solve
+    int y_reset;            // mixed: the FP32 correction accumulator restarts from 0
+    int x_cur;              // mixed: which FP64 solution buffer is current
+    int breakdown;          // mixed: rho = 0 seen before the final update
+    int restart;            // mixed: restart the recurrence (p = r, Rtilde = r) at the next iteration
+    int restarts;           // mixed: restarts taken
+};
+
+enum Exit : int { EXIT_CONVERGED = 0, EXIT_SNORM = 1, EXIT_RHO = 2, EXIT_ITMAX = 3, EXIT_ZERO_ROW = 4 };
+
+constexpr double kTolMin = 2.220446049250313e-16;   // EPSILON(1d0), leq_bicgs.f:277
+constexpr double kSmallNumber = 1.0e-15;            // SMALL_NUMBER, param1_mod.f:24
+constexpr double kRatioTol = (double)0.2f;          // ratiotol = 0.2 (default-real literal), leq_bicgs.f:165
+
+// Plane holding A0: 3 in the reference layout (FP64), 7 in the FP32 difference layout.
+template <typename T> struct DiagPlane { static constexpr int value = 3; };
+template <> struct DiagPlane<float> { static constexpr int value = 7; };
+
+template <typename T> struct Stencil;
+template <> struct Stencil<double> {
+    template <typename X>
+    __device__ __forceinline__ static double row(const DGrid &g, const double *__restrict__ a, const X &x, int64_t c,
+                                                 const Nb &n) {
+        return stencil_row<double>(g, a, x, c, n);
+    }
+};
+template <> struct Stencil<float> {
+    template <typename X>
+    __device__ __forceinline__ static float row(const DGrid &g, const float *__restrict__ a, const X &x, int64_t c,
+                                                const Nb &n) {
+        return stencil_row_diff(g, a, x, c, n);
+    }
+};
+
+// Loop over the cells of logical block BLK (block size BS).
+#define FOR_CELLS(BS, P, BLK, C)                                                    \
+    for (int64_t C = (BLK) * (P).chunk + threadIdx.x,                               \
+                 _end = ((BLK) + 1) * (P).chunk < (P).n ? ((BLK) + 1) * (P).chunk : (P).n; \
+         C < _end; C += (BS))
+
+// Final IER classification after ITMAX trips, leq_bicgs.f:486-495.
+__device__ __forceinline__ void finish_itmax(SolverState *st) {
+    int conv = st->isconverged;
+    const double rn = (double)(float)st->rnorm;
+    if (!conv) conv = rn <= st->tol * st->rnorm0;
+    st->ier = 0;
+    if (!conv) {
+        st->ier = -1;
+        st->iter_tot_inc = st->iter_var;
+        if (rn >= kRatioTol * (double)(float)st->rnorm0) st->ier = -2;
+    }
+    st->isconverged = conv;
+    st->exit_reason = EXIT_ITMAX;
+    st->done = 1;
+}
+
+// ---- scalar logic applied by the finish kernels ---------------------------
+
+enum Fin : int {
+    FIN_SETUP = 0,      // Rnorm0 (leq_bicgs.f:291), zero-row IER=-2 (:217-233)
+    FIN_SHADOW,         // rho_0 (:318) and its zero test at i=1 (:320-332)
+    FIN_ALPHA,          // alpha = rho/<Rtilde,V> (:357-361)
+    FIN_SNORM,          // ||S|| <= EPSILON exit request (:372-374)
+    FIN_OMEGA,          // omega = <T,S>/<T,T>, TxT=0 -> SMALL_NUMBER (:411-429)
+    FIN_XR,             // ||R||, convergence, next rho and beta (:318-343, :439-495)
+    FIN_XR_MIXED,       // same for the FP32 inner recurrence, reliable-update trigger
+    FIN_UPDATE,         // reliable update: FP64 true residual, restart decision
+    FIN_DOT,            // standalone dot product
+};
+
+// True when the phase kernel that wrote the partials returned early (its
+// partials are stale): the finish kernel then does nothing either.
+template <int F>
+__device__ __forceinline__ bool fin_skips(const SolverState *st) {
+    if (F == FIN_SETUP || F == FIN_DOT) return false;
+    if (st->done) return true;
+    if (F == FIN_OMEGA) return st->s_exit != 0;
+    if (F == FIN_UPDATE) return !st->need_update;
+    return false;
+}
+
+template <int F, int NV>
+__device__ __forceinline__ void fin_apply(SolverState *st, const double (&tot)[NV], double *out) {
+    SolverState &S = *st;
+    if (F == FIN_SETUP) {
+        S.rnorm0 = sqrt(tot[0]);
+        S.true_rnorm = S.rnorm0;
+        S.upd_rnorm = S.rnorm0;
+        S.rnorm_max = S.rnorm0;
+        if (S.zero_row) {
+            S.ier = -2;
+            S.exit_reason = EXIT_ZERO_ROW;
+            S.done = 1;
+        }
+    } else if (F == FIN_SHADOW) {
+        S.rho = tot[0];
+        if (S.itmax < 1) {
+            finish_itmax(st);
+        } else if (tot[0] == 0.0) {
+            S.ier = 0;
+            S.exit_reason = EXIT_RHO;
+            S.done = 1;
+        }
+    } else if (F == FIN_ALPHA) {
+        S.rtv = tot[0];
+        S.alpha = S.rho / tot[0];
+    } else if (F == FIN_SNORM) {
+        if (!S.minimize) {
+            S.snorm = sqrt(tot[0]);
+            if (S.snorm <= kTolMin) S.s_exit = 1;
+        }
+    } else if (F == FIN_OMEGA) {
+        double txt = tot[1];
+        S.txs = tot[0];
+        if (txt == 0.0) txt = kSmallNumber;
+        S.txt = txt;
+        S.omega = tot[0] / txt;
+    } else if (F == FIN_XR) {
+        if (S.s_exit) {
+            S.isconverged = 1;
+            S.ier = 0;
+            S.exit_reason = EXIT_SNORM;
+            S.done = 1;
+            return;
+        }
+        const bool checked = !S.minimize || (S.iter_var % S.icheck) == 0;
+        if (checked) {
+            S.rnorm = sqrt(tot[0]);
+            S.isconverged = S.rnorm <= S.tol * S.rnorm0;
+            if (S.isconverged) {
+                S.iter_tot_inc = S.iter_var + 1;
+                S.ier = 0;
+                S.exit_reason = EXIT_CONVERGED;
+                S.done = 1;
+                return;
+            }
+        }
+        S.iter_var += 1;
+        if (S.i >= S.itmax) {
+            finish_itmax(st);
+            return;
+        }
+        S.i += 1;
+        const double rho_new = tot[1];
+        if (rho_new == 0.0) {
+            S.ier = -2;
+            S.exit_reason = EXIT_RHO;
+            S.done = 1;
+            return;
+        }
+        S.beta = (rho_new / S.rho) * (S.alpha / S.omega);
+        S.rho_prev = S.rho;
+        S.rho = rho_new;
+    } else if (F == FIN_XR_MIXED) {
+        S.y_reset = 0;
+        S.restart = 0;
+        if (S.s_exit) {
+            S.need_update = 1;
+            S.final_update = 1;
+            S.exit_reason = EXIT_SNORM;
+            return;
+        }
+        S.rnorm = sqrt(tot[0]);
+        if (S.rnorm > S.rnorm_max) S.rnorm_max = S.rnorm;
+        S.iter_var += 1;
+        if (S.rnorm <= S.tol * S.rnorm0 || S.rnorm <= S.delta * S.upd_rnorm) S.need_update = 1;
+        if (S.i >= S.itmax) {
+            S.need_update = 1;
+            S.final_update = 1;
+            S.exit_reason = EXIT_ITMAX;
+            return;
+        }
+        S.i += 1;
+        const double rho_new = tot[1];
+        if (S.need_update) return;  // the update restarts the recurrence
+        if (rho_new == 0.0) {
+            S.need_update = 1;
+            S.final_update = 1;
+            S.breakdown = 1;
+            S.exit_reason = EXIT_RHO;
+            return;
+        }
+        S.beta = (rho_new / S.rho) * (S.alpha / S.omega);
+        S.rho_prev = S.rho;
+        S.rho = rho_new;
+    } else if (F == FIN_UPDATE) {
+        S.x_cur = 1 - S.x_cur;
+        S.y_reset = 1;
+        S.need_update = 0;
+        S.updates += 1;
+        const double prev_true = S.upd_rnorm;
+        S.true_rnorm = sqrt(tot[0]);
+        S.upd_rnorm = S.true_rnorm;
+        S.rnorm_max = S.true_rnorm;
+        if (S.true_rnorm <= S.tol * S.rnorm0) {
+            S.isconverged = 1;
+            S.ier = 0;
+            S.exit_reason = EXIT_CONVERGED;
+            S.done = 1;
+            return;
+        }
+        if (S.final_update) {
+            S.ier = S.true_rnorm < kRatioTol * S.rnorm0 ? -1 : -2;
+            if (S.breakdown) {
+                S.ier = -2;
+                S.exit_reason = EXIT_RHO;
+            }
+            S.done = 1;
+            return;
+        }
+        // Continue the recurrence from the replaced residual while the
+        // updates make progress; restart it (p = r, Rtilde = r, rho = <r,r>)
+        // when the true residual grew since the last update or the FP32
+        // recurrence drifted more than 10x away from it.
+        const bool drift = S.true_rnorm > prev_true || S.true_rnorm > 10.0 * S.rnorm;
+        const double rho_cont = tot[1];
+        if (drift || rho_cont == 0.0) {
+            S.restart = 1;
+            S.restarts += 1;
+            S.rho = tot[2];
+            if (S.rho == 0.0) {
+                S.ier = -2;
+                S.exit_reason = EXIT_RHO;
+                S.done = 1;
+            }
+        } else {
+            S.beta = (rho_cont / S.rho) * (S.alpha / S.omega);
+            S.rho_prev = S.rho;
+            S.rho = rho_cont;
+        }
+    } else if (F == FIN_DOT) {
+        *out = tot[0];
+    }
+}
+
+template <int F, int NV>
+__global__ void __launch_bounds__(kFinishBlock) k_finish(SolverState *st, const double *__restrict__ partials,
+                                                         int64_t nblk, double *out) {
+    __shared__ double lds[(kFinishBlock / 64) * 4];
+    if (fin_skips<F>(st)) return;
+    double tot[NV];
+    finish_sum<NV>(partials, nblk, lds, tot);
+    if (threadIdx.x == 0) fin_apply<F, NV>(st, tot, out);
+}
+
+// ---- LINE preconditioner (LEQ_MSOLVE, leqsol_mod.f:286-598) ---------------
+//
+// One thread solves one line with the DGTSV algorithm of model/DGTSV.f:107-163
+// (elimination with partial pivoting) and :233-246 (back solve), working in
+// four scratch planes indexed like the cells of the line, so the result is
+// the reference's bit for bit.  Lines of one phase are independent (red-black
+// colours, anti-diagonal wavefronts), so a phase is one launch.
+
+enum LineAxis : int { AX_J = 0, AX_I = 1, AX_K = 2 };
+enum PhaseKind : int { PH_COLOR = 0, PH_DIAG = 1, PH_SEQ = 2 };
+
+struct LinePhase {
+    int axis;        // direction of the lines
+    int kind;        // PH_COLOR: (u+v*vw) parity == param; PH_DIAG: u+v == param; PH_SEQ: reference order
+    int param;
+    int vw;          // weight of v in the colour (0 for the 2-D I-colouring)
+    int fatal;       // INFO>0 is fatal (JK/IJ sweeps) instead of leaving the line unchanged
+    int seq_s1;      // PH_SEQ: fast size and pass (icase) of the reference enumeration
+    int seq_icase;
+};
+
+// Axis geometry of a line family: length n, stride along the line, and the
+// (u, v) coordinates that label the lines (1-based like the reference).
+struct LineGeom {
+    int n, nu, nv;
+    int64_t stride, su, sv;
+};
+
+__host__ __device__ inline LineGeom line_geom(const DGrid &g, int axis) {
+    LineGeom L;
+    if (axis == AX_J) { L.n = g.nj; L.stride = 1; L.nu = g.ni; L.su = g.c1; L.nv = g.nk; L.sv = g.c2; }
+    else if (axis == AX_I) { L.n = g.ni; L.stride = g.c1; L.nu = g.nj; L.su = 1; L.nv = g.nk; L.sv = g.c2; }
+    else { L.n = g.nk; L.stride = g.c2; L.nu = g.ni; L.su = g.c1; L.nv = g.nj; L.sv = 1; }
+    return L;
+}
+
+// Position p (1-based) of the reference's red-black enumeration of a 2-D
+// index set (leqsol_mod.f:404-418) -> 1-based (fast, slow).
+__device__ __forceinline__ void rb_position(int p, int s1, int *f, int *s) {
+    int slow = (p % s1 != 0) ? p / s1 + 1 : p / s1;
+    int fast = (p - 1 - (slow - 1) * s1) + 1 + (slow % 2);
+    if (fast > s1) fast = fast - s1;
+    *f = fast;
+    *s = slow;
+}
+
+template <typename T>
+__device__ void solve_line(const DGrid &g, const LineGeom &L, int axis, int64_t c0, const T *__restrict__ a,
+                           const T *__restrict__ bm, T *__restrict__ var, T *__restrict__ sdl,
+                           T *__restrict__ sd, T *__restrict__ sdu, T *__restrict__ sb, int fatal,
+                           int *fatal_flag) {
+    const int64_t n3 = g.n3;
+    const int n = L.n;
+    const int pd = DiagPlane<T>::value;
+    // plane indices: lower/upper along the line; the two cross axes
+    int plo, pup, x1m, x1p, x2m, x2p;
+    if (axis == AX_J) { plo = 1; pup = 5; x1m = 2; x1p = 4; x2m = 0; x2p = 6; }
+    else if (axis == AX_I) { plo = 2; pup = 4; x1m = 1; x1p = 5; x2m = 0; x2p = 6; }
+    else { plo = 0; pup = 6; x1m = 1; x1p = 5; x2m = 2; x2p = 4; }
+    // build DD, CC, EE, BB (LEQ_IKSWEEP leqsol_mod.f:891-902 and its JK/IJ twins)
+    for (int q = 0; q < n; ++q) {
+        const int64_t c = c0 + (int64_t)q * L.stride;
+        const Nb nb = neighbours(g, c);
+        int64_t m1, p1, m2, p2;
+        if (axis == AX_J) { m1 = nb.im; p1 = nb.ip; m2 = nb.km; p2 = nb.kp; }
+        else if (axis == AX_I) { m1 = nb.jm; p1 = nb.jp; m2 = nb.km; p2 = nb.kp; }
+        else { m1 = nb.jm; p1 = nb.jp; m2 = nb.im; p2 = nb.ip; }
+        T bb;
+        if (axis == AX_J && !g.do_k)
+            bb = bm[c] - a[x1m * n3 + c] * var[m1] - a[x1p * n3 + c] * var[p1];
+        else
+            bb = bm[c] - a[x1m * n3 + c] * var[m1] - a[x1p * n3 + c] * var[p1] - a[x2m * n3 + c] * var[m2] -
+                 a[x2p * n3 + c] * var[p2];
+        sd[c] = a[pd * n3 + c];
+        sdl[c] = (q == 0) ? (T)0 : a[plo * n3 + c];     // CC(NSTART) = 0
+        sdu[c] = (q == n - 1) ? (T)0 : a[pup * n3 + c]; // EE(NEND) = 0
+        sb[c] = bb;
+    }
+    // DGTSV.f:107-163; DL(i) is CC of row i+1, stored at that row's cell
+#define AT(q) (c0 + (int64_t)(q) * L.stride)
+    int info = 0;
+    for (int q = 0; q < n - 2 && !info; ++q) {
+        const T dq = sd[AT(q)], dlq = sdl[AT(q + 1)];
+        if (fabs((double)dq) >= fabs((double)dlq)) {
+            if (dq != (T)0) {
+                const T fact = dlq / dq;
+                sd[AT(q + 1)] = sd[AT(q + 1)] - fact * sdu[AT(q)];
+                sb[AT(q + 1)] = sb[AT(q + 1)] - fact * sb[AT(q)];
+            } else {
+                info = q + 1;
+                break;
+            }
+            sdl[AT(q + 1)] = (T)0;
+        } else {
+            const T fact = dq / dlq;
+            sd[AT(q)] = dlq;
+            T temp = sd[AT(q + 1)];
+            sd[AT(q + 1)] = sdu[AT(q)] - fact * temp;
+            sdl[AT(q + 1)] = sdu[AT(q + 1)];
+            sdu[AT(q + 1)] = -fact * sdl[AT(q + 1)];
+            sdu[AT(q)] = temp;
+            temp = sb[AT(q)];
+            sb[AT(q)] = sb[AT(q + 1)];
+            sb[AT(q + 1)] = temp - fact * sb[AT(q + 1)];
+        }
+    }
+    if (!info && n > 1) {
+        const int q = n - 2;
+        const T dq = sd[AT(q)], dlq = sdl[AT(q + 1)];
+        if (fabs((double)dq) >= fabs((double)dlq)) {
+            if (dq != (T)0) {
+                const T fact = dlq / dq;
+                sd[AT(q + 1)] = sd[AT(q + 1)] - fact * sdu[AT(q)];
+                sb[AT(q + 1)] = sb[AT(q + 1)] - fact * sb[AT(q)];
+            } else {
+                info = q + 1;
+            }
+        } else {
+            const T fact = dq / dlq;
+            sd[AT(q)] = dlq;
+            T temp = sd[AT(q + 1)];
+            sd[AT(q + 1)] = sdu[AT(q)] - fact * temp;
+            sdu[AT(q)] = temp;
+            temp = sb[AT(q)];
+            sb[AT(q)] = sb[AT(q + 1)];
+            sb[AT(q + 1)] = temp - fact * sb[AT(q + 1)];
+        }
+    }
+    if (!info && sd[AT(n - 1)] == (T)0) info = n;
+    if (info) {
+        if (fatal) atomicOr(fatal_flag, 1);
+        return;  // the line keeps its current values (leqsol_mod.f:910-914)
+    }
+    // back solve, DGTSV.f:233-246 (DL(i) here is the fill-in of row i+1)
+    sb[AT(n - 1)] = sb[AT(n - 1)] / sd[AT(n - 1)];
+    if (n > 1) sb[AT(n - 2)] = (sb[AT(n - 2)] - sdu[AT(n - 2)] * sb[AT(n - 1)]) / sd[AT(n - 2)];
+    for (int q = n - 3; q >= 0; --q)
+        sb[AT(q)] = (sb[AT(q)] - sdu[AT(q)] * sb[AT(q + 1)] - sdl[AT(q + 1)] * sb[AT(q + 2)]) / sd[AT(q)];
+    for (int q = 0; q < n; ++q) var[AT(q)] = sb[AT(q)];
+#undef AT
+}
+
+// One phase of line solves.  `st` (nullable) makes the launch a no-op once
+// the solve is done, or, with skip_on_sexit, after the S-norm exit.
+template <typename T>
+__global__ void __launch_bounds__(64) k_line_phase(DGrid g, LinePhase ph, const T *__restrict__ a,
+                                                   const T *__restrict__ bm, T *__restrict__ var, T *sdl, T *sd,
+                                                   T *sdu, T *sb, const SolverState *st, int skip_on_sexit,
+                                                   int *fatal_flag) {
+    if (st) {
+        if (st->done) return;
+        if (skip_on_sexit && st->s_exit) return;
+    }
+    const LineGeom L = line_geom(g, ph.axis);
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ph.kind == PH_SEQ) {
+        // reference enumeration of one red-black pass when the colouring is
+        // not proper (odd fast size): one thread walks the lines in order
+        if (tid != 0) return;
+        const int total = L.nu * L.nv;
+        for (int p = ph.seq_icase; p <= total; p += 2) {
+            int f, s;
+            rb_position(p, ph.seq_s1, &f, &s);
+            const int64_t c0 = (int64_t)(f - 1) * L.su + (int64_t)(s - 1) * L.sv;
+            solve_line<T>(g, L, ph.axis, c0, a, bm, var, sdl, sd, sdu, sb, ph.fatal, fatal_flag);
+        }
+        return;
+    }
+    int u, v;  // 0-based line coordinates
+    if (ph.kind == PH_COLOR) {
+        if (tid >= (int64_t)L.nu * L.nv) return;
+        u = (int)(tid % L.nu);
+        v = (int)(tid / L.nu);
+        if ((((u + 1) + (v + 1) * ph.vw) & 1) != ph.param) return;
+    } else {
+        const int d = ph.param;
+        const int lo = d - (L.nv - 1) > 0 ? d - (L.nv - 1) : 0;
+        const int hi = d < L.nu - 1 ? d : L.nu - 1;
+        if (tid > hi - lo) return;
+        u = lo + (int)tid;
+        v = d - u;
+    }
+    const int64_t c0 = (int64_t)u * L.su + (int64_t)v * L.sv;
+    solve_line<T>(g, L, ph.axis, c0, a, bm, var, sdl, sd, sdu, sb, ph.fatal, fatal_flag);
+}
+
+}  // namespace mfix

@@ -1,0 +1,231 @@
+"""Host-side mirror of the reference's LEQ interface, backed by the HIP library.
+
+``leq_bicgs`` and ``solve_lin_eq`` take the reference's arguments in the
+reference's order (model/leq_bicgs.f:23-24, model/solve_lin_eq.f:17-18) and
+act on numpy arrays in the reference's serial storage; like the reference
+they overwrite VAR with the solution and A_M/B_M with the row-scaled system
+and return IER (0, -1 or -2).  An unknown PC raises, where the reference calls
+mfix_exit (model/leq_bicgs.f:83-87).
+
+``DeviceArray`` and the ``*_dev`` functions keep A_M/B_M/VAR resident in HBM
+for repeated solves; every call goes through libmfix_leq.so, and nothing in
+this module computes a result on the host.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import Grid, Opts, Stats, check
+
+
+class SolverAbort(RuntimeError):
+    """Raised where the reference aborts through mfix_exit."""
+
+
+def _lib_():
+    return _lib.load()
+
+
+def init(device: int = 0) -> None:
+    check(_lib_().mfix_leq_init(device), "mfix_leq_init")
+
+
+def finalize() -> None:
+    check(_lib_().mfix_leq_finalize(), "mfix_leq_finalize")
+
+
+def grid_init(imax: int, jmax: int, kmax: int, *, no_k: bool = False, cyclic=(False, False, False),
+              is_serial: bool = True, minimize_dotproducts: bool = False, icheck_bicgs: int = 1) -> Grid:
+    """Replaces the GRIDMAP_INIT/SET_INCREMENTS module state the solver reads."""
+    g = Grid(imax, jmax, kmax, int(no_k), int(cyclic[0]), int(cyclic[1]), int(cyclic[2]),
+             int(is_serial), int(minimize_dotproducts), int(icheck_bicgs))
+    check(_lib_().mfix_leq_grid_init(C.byref(g)), "mfix_leq_grid_init")
+    return g
+
+
+def dimension_3() -> int:
+    return check(_lib_().mfix_leq_dimension_3(), "mfix_leq_dimension_3")
+
+
+def iter_tot(vno: int) -> int:
+    return check(_lib_().mfix_leq_iter_tot(vno), "mfix_leq_iter_tot")
+
+
+def reset_iter_tot() -> None:
+    _lib_().mfix_leq_reset_iter_tot()
+
+
+def _dp(a: np.ndarray):
+    if a.dtype != np.float64 or not a.flags.c_contiguous or not a.flags.writeable:
+        raise TypeError("arrays must be writable, C-contiguous float64")
+    return a.ctypes.data_as(_lib.c_double_p)
+
+
+def set_noise(noise: np.ndarray | None, seed: int = 0) -> None:
+    """Inject the U[0,1) array used for Rtilde by ``leq_bicgs`` (parity runs)."""
+    global _noise_keepalive
+    _noise_keepalive = None if noise is None else np.ascontiguousarray(noise, dtype=np.float64)
+    ptr = None if noise is None else _noise_keepalive.ctypes.data
+    check(_lib_().mfix_leq_set_noise(ptr, C.c_uint64(seed)), "mfix_leq_set_noise")
+
+
+_noise_keepalive = None
+
+
+def leq_bicgs(vname: str, vno: int, var: np.ndarray, a_m: np.ndarray, b_m: np.ndarray, cmethod: str,
+              tol: float, pc: str, itmax: int) -> int:
+    """LEQ_BICGS (model/leq_bicgs.f:23-90) on host arrays; returns IER."""
+    ier = C.c_int(0)
+    n3 = dimension_3()
+    if var.size != n3 or a_m.size != 7 * n3 or b_m.size != n3:
+        raise ValueError(f"array sizes do not match DIMENSION_3={n3}")
+    vb = vname.encode()
+    cb = cmethod.encode()
+    rc = _lib_().mfix_leq_bicgs(vb, len(vb), vno, _dp(var), _dp(a_m), _dp(b_m), cb, len(cb), float(tol),
+                                pc.encode().ljust(4)[:4], int(itmax), C.byref(ier))
+    if rc == -101:
+        raise SolverAbort("preconditioner option not found - check mfix.dat and readme")
+    if rc == -106:
+        raise SolverAbort("DGTSV failed in a JK/IJ line sweep")
+    check(rc, "mfix_leq_bicgs")
+    return ier.value
+
+
+def last_stats() -> dict:
+    st = Stats()
+    check(_lib_().mfix_leq_last_stats(C.byref(st)), "mfix_leq_last_stats")
+    return st.as_dict()
+
+
+def solve_lin_eq(vname: str, vno: int, var: np.ndarray, a_m: np.ndarray, b_m: np.ndarray, m: int,
+                 itmax: int, method: int, sweep: str, tol1: float, pc: str) -> int:
+    """SOLVE_LIN_EQ (model/solve_lin_eq.f:17-195): a_m is (M+1, 7, N3), b_m (M+1, N3)."""
+    ier = C.c_int(0)
+    vb = vname.encode()
+    rc = _lib_().mfix_solve_lin_eq(vb, len(vb), vno, _dp(var), _dp(a_m), _dp(b_m), int(m), int(itmax),
+                                   int(method), sweep.encode().ljust(4)[:4], float(tol1),
+                                   pc.encode().ljust(4)[:4], C.byref(ier))
+    if rc == -101:
+        raise SolverAbort("preconditioner option not found - check mfix.dat and readme")
+    if rc == -108:
+        raise SolverAbort(f"LEQ_METHOD = {method} is not provided by this library")
+    check(rc, "mfix_solve_lin_eq")
+    return ier.value
+
+
+# ---- device-resident arrays ---------------------------------------------------
+
+class DeviceArray:
+    """A block of HBM owned by libmfix_leq (float64 or float32 elements)."""
+
+    def __init__(self, n: int, dtype=np.float64):
+        self.n = int(n)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = self.n * self.dtype.itemsize
+        p = C.c_void_p()
+        check(_lib_().mfix_leq_dev_alloc(self.nbytes, C.byref(p)), "mfix_leq_dev_alloc")
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(a.size, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.size == self.n
+        check(_lib_().mfix_leq_copy_to_dev(self.ptr, a.ctypes.data, self.nbytes), "copy_to_dev")
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.n, dtype=self.dtype)
+        check(_lib_().mfix_leq_copy_from_dev(out.ctypes.data, self.ptr, self.nbytes), "copy_from_dev")
+        return out
+
+    def copy_from(self, other: "DeviceArray") -> None:
+        assert other.nbytes == self.nbytes
+        check(_lib_().mfix_leq_copy_dev(self.ptr, other.ptr, self.nbytes), "copy_dev")
+
+    def free(self) -> None:
+        if self.ptr:
+            _lib_().mfix_leq_dev_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):  # pragma: no cover - interpreter shutdown order
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+@dataclass
+class SolveResult:
+    ier: int
+    stats: dict
+
+
+def _opts(noise: DeviceArray | None, seed: int) -> Opts:
+    o = Opts()
+    o.write_back_scaled = 1
+    o.noise = noise.ptr if noise is not None else None
+    o.noise_seed = seed
+    return o
+
+
+def bicgs_dev(var: DeviceArray, a_m: DeviceArray, b_m: DeviceArray, *, pc: str = "DIAG",
+              cmethod: str = "RSRS", tol: float = 1e-10, itmax: int = 500, noise: DeviceArray | None = None,
+              seed: int = 0, mixed: bool = False) -> SolveResult:
+    """FP64 (or mixed FP32/FP64) LEQ_BICGS on resident device arrays."""
+    st = Stats()
+    o = _opts(noise, seed)
+    fn = _lib_().mfix_leq_bicgs_mixed_dev if mixed else _lib_().mfix_leq_bicgs_dev
+    rc = fn(cmethod.encode().ljust(4)[:4], float(tol), pc.encode().ljust(4)[:4], int(itmax), var.ptr, a_m.ptr,
+            b_m.ptr, C.byref(o), C.byref(st), None)
+    if rc == -101:
+        raise SolverAbort("preconditioner option not found - check mfix.dat and readme")
+    check(rc, "bicgs_dev")
+    return SolveResult(st.ier, st.as_dict())
+
+
+def matvec_dev(a_m: DeviceArray, x: DeviceArray, y: DeviceArray) -> None:
+    if a_m.dtype == np.float32:
+        check(_lib_().mfix_leq_matvec32_dev(a_m.ptr, x.ptr, y.ptr, None), "matvec32")
+    else:
+        check(_lib_().mfix_leq_matvec_dev(a_m.ptr, x.ptr, y.ptr, None), "matvec")
+
+
+def msolve_dev(pc: str, cmethod: str, a_m: DeviceArray, r: DeviceArray, z: DeviceArray) -> int:
+    rc = _lib_().mfix_leq_msolve_dev(pc.encode().ljust(4)[:4], cmethod.encode().ljust(4)[:4], a_m.ptr, r.ptr,
+                                     z.ptr, None)
+    if rc == -106:
+        raise SolverAbort("DGTSV failed in a JK/IJ line sweep")
+    return check(rc, "msolve")
+
+
+def dot_dev(x: DeviceArray, y: DeviceArray) -> float:
+    out = C.c_double()
+    check(_lib_().mfix_leq_dot_dev(x.ptr, y.ptr, C.byref(out), None), "dot")
+    return out.value
+
+
+def scale_dev(a_m: DeviceArray, b_m: DeviceArray) -> int:
+    ier = C.c_int()
+    check(_lib_().mfix_leq_scale_dev(a_m.ptr, b_m.ptr, C.byref(ier), None), "scale")
+    return ier.value
+
+
+def prof_enable(which: int, on: bool = True) -> None:
+    check(_lib_().mfix_leq_prof_enable(which, int(on)), "prof_enable")
+
+
+def prof_read(which: int) -> dict:
+    n = C.c_int64()
+    ms = C.c_double()
+    by = C.c_double()
+    check(_lib_().mfix_leq_prof_read(which, C.byref(n), C.byref(ms), C.byref(by)), "prof_read")
+    return {"launches": n.value, "total_ms": ms.value, "alg_bytes": by.value}

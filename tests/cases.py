@@ -1,0 +1,83 @@
+"""Shared test cases: small systems covering the layout's corner cases.
+
+Grids are deliberately non-cubic (a transposed stride or axis fails them),
+include odd and even sizes (the red-black and ASAS colourings differ), cyclic
+axes (wrapped increments, model/set_increments.f:72-112) and 2-D runs (NO_K).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from mfix_amd import systems
+
+# (name, kind, imax, jmax, kmax, no_k, cyclic)
+GRIDS = [
+    ("p_6x5x4", 0, 6, 5, 4, False, (0, 0, 0)),
+    ("m_7x6x5", 1, 7, 6, 5, False, (0, 0, 0)),
+    ("m_9x8x6_cyc", 1, 9, 8, 6, False, (1, 0, 1)),
+    ("p_8x7x6_cycy", 0, 8, 7, 6, False, (0, 1, 0)),
+    ("v_12x10x9", 2, 12, 10, 9, False, (0, 0, 0)),
+    ("m_10x8_2d", 1, 10, 8, 1, True, (0, 0, 0)),
+]
+
+
+def make_system(kind, imax, jmax, kmax, no_k=False, seed=7):
+    """A_M/B_M in the serial layout; 2-D systems drop the K planes' couplings."""
+    if not no_k:
+        return systems.gen_system(kind, imax, jmax, kmax, seed, nspheres=6)
+    # a 2-D run: one K layer.  Build a 3-D system with kmax=1 and keep its
+    # middle layer, zeroing the K couplings and moving them into the diagonal
+    a3, b3 = systems.gen_system(kind, imax, jmax, 1, seed, nspheres=6)
+    ni, nj = imax + 2, jmax + 2
+    n3d = ni * nj * 3
+    a3 = a3.reshape(7, n3d)
+    layer = slice(ni * nj, 2 * ni * nj)
+    a = a3[:, layer].copy()
+    a[3] += a[0] + a[6]
+    a[0] = 0.0
+    a[6] = 0.0
+    return a.reshape(-1).copy(), b3[layer].copy()
+
+
+def to_csr(a, imax, jmax, kmax, no_k=False, cyclic=(0, 0, 0)):
+    """The 7-point system as a scipy CSR matrix, built independently of the
+    oracle with numpy index arithmetic: neighbours clamp to the cell itself
+    at the array ends and wrap on interior cells of cyclic axes
+    (model/set_increments.f:72-112)."""
+    import scipy.sparse as sp
+    ni, nj, nk = imax + 2, jmax + 2, (1 if no_k else kmax + 2)
+    n3 = ni * nj * nk
+    k, i, j = np.meshgrid(np.arange(nk), np.arange(ni), np.arange(nj), indexing="ij")
+    k, i, j = k.ravel(), i.ravel(), j.ravel()
+
+    def shift(v, n, d, cyc):
+        t = v + d
+        inner = (v > 0) & (v < n - 1)
+        if cyc:
+            wrapped = np.where(t == 0, n - 2, np.where(t == n - 1, 1, t))
+            return np.where(inner, wrapped, np.clip(t, 0, n - 1))
+        return np.clip(t, 0, n - 1)
+
+    idx = lambda ii, jj, kk: jj + ii * nj + kk * nj * ni
+    rows = np.arange(n3)
+    planes = a.reshape(7, n3)
+    cols = {
+        3: rows,
+        1: idx(i, shift(j, nj, -1, cyclic[1]), k), 5: idx(i, shift(j, nj, 1, cyclic[1]), k),
+        2: idx(shift(i, ni, -1, cyclic[0]), j, k), 4: idx(shift(i, ni, 1, cyclic[0]), j, k),
+    }
+    if not no_k:
+        cols[0] = idx(i, j, shift(k, nk, -1, cyclic[2]))
+        cols[6] = idx(i, j, shift(k, nk, 1, cyclic[2]))
+    r = np.concatenate([rows for _ in cols])
+    c = np.concatenate([cols[p] for p in cols])
+    v = np.concatenate([planes[p] for p in cols])
+    return sp.csr_matrix((v, (r, c)), shape=(n3, n3))
+
+
+def n3_of(imax, jmax, kmax, no_k):
+    return (imax + 2) * (jmax + 2) * (1 if no_k else kmax + 2)
+
+
+def rng_vec(n, seed):
+    return np.random.default_rng(seed).uniform(-1.0, 1.0, n)

@@ -1,0 +1,128 @@
+"""GPU parity of the path's kernels against the oracle (through the C ABI).
+
+LEQ_MATVEC, LEQ_MSOLVE1 (DIAG), LEQ_MSOLVE (LINE; every sweep mode) and the
+unit scaling are element-wise or line-sequential computations that the HIP
+kernels evaluate in the reference's operation order without contraction, so
+they must match the oracle bit for bit.  dot_product_par is a reduction whose
+order differs (tree vs. sequential); it is checked against an exact sum.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from tests.cases import GRIDS, make_system, n3_of, rng_vec
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(gpu, oracle_mod, kind, im, jm, km, no_k, cyc, **kw):
+    gpu.grid_init(im, jm, km, no_k=no_k, cyclic=tuple(bool(c) for c in cyc), **kw)
+    og = oracle_mod.grid(im, jm, km, no_k=no_k, cyclic=cyc, **kw)
+    a, b = make_system(kind, im, jm, km, no_k)
+    return og, a, b
+
+
+@pytest.mark.parametrize("case", GRIDS, ids=[c[0] for c in GRIDS])
+def test_matvec_bit_exact(gpu, oracle_mod, case):
+    name, kind, im, jm, km, no_k, cyc = case
+    og, a, b = _setup(gpu, oracle_mod, kind, im, jm, km, no_k, cyc)
+    x = rng_vec(a.size // 7, 3)
+    da, dx = gpu.DeviceArray.from_host(a), gpu.DeviceArray.from_host(x)
+    dy = gpu.DeviceArray(x.size)
+    gpu.matvec_dev(da, dx, dy)
+    np.testing.assert_array_equal(dy.download(), oracle_mod.matvec(og, a, x))
+
+
+@pytest.mark.parametrize("case", GRIDS, ids=[c[0] for c in GRIDS])
+def test_unit_scaling_bit_exact(gpu, oracle_mod, case):
+    name, kind, im, jm, km, no_k, cyc = case
+    og, a, b = _setup(gpu, oracle_mod, kind, im, jm, km, no_k, cyc)
+    a_ref, b_ref, ier_ref = oracle_mod.unit_scale(og, a, b)
+    da, db = gpu.DeviceArray.from_host(a), gpu.DeviceArray.from_host(b)
+    assert gpu.scale_dev(da, db) == ier_ref == 0
+    np.testing.assert_array_equal(da.download(), a_ref)
+    np.testing.assert_array_equal(db.download(), b_ref)
+
+
+def test_unit_scaling_zero_row(gpu, oracle_mod):
+    og, a, b = _setup(gpu, oracle_mod, 0, 5, 4, 3, False, (0, 0, 0))
+    n3 = b.size
+    a.reshape(7, n3)[:, 17] = 0.0
+    _, _, ier_ref = oracle_mod.unit_scale(og, a, b)
+    da, db = gpu.DeviceArray.from_host(a), gpu.DeviceArray.from_host(b)
+    assert ier_ref == -2
+    assert gpu.scale_dev(da, db) == -2
+
+
+@pytest.mark.parametrize("case", GRIDS, ids=[c[0] for c in GRIDS])
+def test_diag_bit_exact(gpu, oracle_mod, case):
+    name, kind, im, jm, km, no_k, cyc = case
+    og, a, b = _setup(gpu, oracle_mod, kind, im, jm, km, no_k, cyc)
+    r = rng_vec(b.size, 5)
+    da, dr = gpu.DeviceArray.from_host(a), gpu.DeviceArray.from_host(r)
+    dz = gpu.DeviceArray(r.size)
+    gpu.msolve_dev("DIAG", "RSRS", da, dr, dz)
+    np.testing.assert_array_equal(dz.download(), oracle_mod.msolve(og, "DIAG", a, r)[0])
+
+
+LINE_CASES = [(c, m) for c in GRIDS for m in ("RSRS", "ISIS", "ASAS", "KSKS", "RIRS", "JSJS")]
+
+
+@pytest.mark.parametrize("case,cmethod", LINE_CASES, ids=[f"{c[0]}-{m}" for c, m in LINE_CASES])
+def test_line_msolve_bit_exact(gpu, oracle_mod, case, cmethod):
+    name, kind, im, jm, km, no_k, cyc = case
+    og, a, b = _setup(gpu, oracle_mod, kind, im, jm, km, no_k, cyc)
+    # the solver applies LINE to the unit-scaled system
+    a, b, _ = oracle_mod.unit_scale(og, a, b)
+    r = rng_vec(b.size, 11)
+    z_ref, info = oracle_mod.msolve(og, "LINE", a, r, cmethod)
+    assert info == 0
+    da, dr = gpu.DeviceArray.from_host(a), gpu.DeviceArray.from_host(r)
+    dz = gpu.DeviceArray(r.size)
+    gpu.msolve_dev("LINE", cmethod, da, dr, dz)
+    np.testing.assert_array_equal(dz.download(), z_ref)
+
+
+def test_line_msolve_pivoting_lines(gpu, oracle_mod):
+    """Lines that are not diagonally dominant take DGTSV's row interchanges."""
+    og, a, b = _setup(gpu, oracle_mod, 1, 7, 9, 5, False, (0, 0, 0))
+    n3 = b.size
+    a2 = a.reshape(7, n3)
+    rng = np.random.default_rng(1)
+    # inflate the J couplings on a third of the cells so |D| < |DL| somewhere
+    sel = rng.random(n3) < 0.33
+    a2[1, sel] *= 25.0
+    a2[5, sel] *= 25.0
+    r = rng_vec(n3, 2)
+    for cm in ("RSRS", "ASAS"):
+        z_ref, info = oracle_mod.msolve(og, "LINE", a, r, cm)
+        da, dr = gpu.DeviceArray.from_host(a), gpu.DeviceArray.from_host(r)
+        dz = gpu.DeviceArray(n3)
+        gpu.msolve_dev("LINE", cm, da, dr, dz)
+        np.testing.assert_array_equal(dz.download(), z_ref)
+
+
+@pytest.mark.parametrize("serial", [True, False])
+@pytest.mark.parametrize("case", GRIDS[:3] + GRIDS[5:], ids=[c[0] for c in GRIDS[:3] + GRIDS[5:]])
+def test_dot_product_par(gpu, oracle_mod, case, serial):
+    name, kind, im, jm, km, no_k, cyc = case
+    og, a, b = _setup(gpu, oracle_mod, kind, im, jm, km, no_k, cyc, is_serial=serial)
+    x, y = rng_vec(b.size, 1), rng_vec(b.size, 2)
+    dx, dy = gpu.DeviceArray.from_host(x), gpu.DeviceArray.from_host(y)
+    got = gpu.dot_dev(dx, dy)
+    ref = oracle_mod.dot(og, x, y)
+    # exact value of the same domain, for a tolerance independent of order
+    if serial:
+        exact = math.fsum(x * y)
+    else:
+        ni, nj, nk = im + 2, jm + 2, (1 if no_k else km + 2)
+        m = np.zeros((nk, ni, nj), bool)
+        if no_k:
+            m[0, 1:-1, 1:-1] = True
+        else:
+            m[1:-1, 1:-1, 1:-1] = True
+        exact = math.fsum((x * y)[m.reshape(-1)])
+    bound = 1e-14 * np.sum(np.abs(x * y))
+    assert abs(got - exact) <= bound
+    assert abs(ref - exact) <= bound
