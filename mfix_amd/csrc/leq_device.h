@@ -264,29 +264,53 @@ __device__ __forceinline__ void block_partial(double (&v)[NV], double *partials,
     }
 }
 
-// Inside a kFinishBlock-thread finish kernel: totals of the NV partial arrays
-// (valid in thread 0), summed in a fixed order.  Loads are issued in groups
-// of four so a thread's partials arrive together instead of one round trip
-// each.
+// Finish step, run by kFinishBlocks blocks of kFinishBlock threads.  Block
+// b sums the partials of slice b in a fixed order into level-2 sums; the
+// last block to arrive (counted with one device-scope atomic per block)
+// adds the level-2 sums in slice order, so the result does not depend on
+// arrival order.  Only the few level-2 values are written before the
+// release fence, so the fence has almost nothing to write back.  Returns
+// true in the last block, with the totals in thread 0; the counter is
+// reset there for the next launch.
+constexpr int kFinishBlocks = 64;
+
+__device__ __forceinline__ double load_agent(const double *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 template <int NV>
-__device__ __forceinline__ void finish_sum(const double *partials, int64_t nblk, double *lds, double (&tot)[NV]) {
+__device__ __forceinline__ bool finish_sum(const double *partials, int64_t nblk, double *l2, unsigned *counter,
+                                           double *lds, double (&tot)[NV]) {
+    __shared__ bool last;
+    const int64_t per = (nblk + gridDim.x - 1) / gridDim.x;
+    const int64_t lo = (int64_t)blockIdx.x * per;
+    const int64_t hi = lo + per < nblk ? lo + per : nblk;
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
         const double *p = partials + q * nblk;
         double s = 0.0;
-        int64_t b = threadIdx.x;
-        for (; b + 3 * kFinishBlock < nblk; b += 4 * kFinishBlock) {
-            const double v0 = p[b], v1 = p[b + kFinishBlock], v2 = p[b + 2 * kFinishBlock],
-                         v3 = p[b + 3 * kFinishBlock];
-            s += v0;
-            s += v1;
-            s += v2;
-            s += v3;
-        }
-        for (; b < nblk; b += kFinishBlock) s += p[b];
+        for (int64_t b = lo + threadIdx.x; b < hi; b += kFinishBlock) s += p[b];
         tot[q] = s;
     }
     block_sum<kFinishBlock, NV>(tot, lds);
+    if (gridDim.x == 1) return true;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) l2[q * gridDim.x + blockIdx.x] = tot[q];
+        __threadfence();
+        const unsigned prev = atomicAdd(counter, 1u);
+        last = prev == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return false;
+    __threadfence();
+#pragma unroll
+    for (int q = 0; q < NV; ++q)
+        tot[q] = threadIdx.x < gridDim.x ? load_agent(&l2[q * gridDim.x + threadIdx.x]) : 0.0;
+    __syncthreads();
+    block_sum<kFinishBlock, NV>(tot, lds);
+    if (threadIdx.x == 0) *counter = 0u;
+    return true;
 }
 
 // Counter-based U[0,1) (same construction as the host generator).

@@ -49,6 +49,9 @@ struct SolverState {
     int breakdown;          // mixed: rho = 0 seen before the final update
     int restart;            // mixed: restart the recurrence (p = r, Rtilde = r) at the next iteration
     int restarts;           // mixed: restarts taken
+    unsigned fin_counter;   // arrivals of the current finish launch (reset by its last block)
+    int pad_;
+    double l2[4 * kFinishBlocks];  // level-2 sums of the finish launches
 };
 
 enum Exit : int { EXIT_CONVERGED = 0, EXIT_SNORM = 1, EXIT_RHO = 2, EXIT_ITMAX = 3, EXIT_ZERO_ROW = 4 };
@@ -284,8 +287,8 @@ __global__ void __launch_bounds__(kFinishBlock) k_finish(SolverState *st, const 
     __shared__ double lds[(kFinishBlock / 64) * 4];
     if (fin_skips<F>(st)) return;
     double tot[NV];
-    finish_sum<NV>(partials, nblk, lds, tot);
-    if (threadIdx.x == 0) fin_apply<F, NV>(st, tot, out);
+    if (finish_sum<NV>(partials, nblk, st->l2, &st->fin_counter, lds, tot) && threadIdx.x == 0)
+        fin_apply<F, NV>(st, tot, out);
 }
 
 // ---- LINE preconditioner (LEQ_MSOLVE, leqsol_mod.f:286-598) ---------------

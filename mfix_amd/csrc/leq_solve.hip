@@ -176,7 +176,11 @@ static inline dim3 grid_of(const Part &P) { return dim3((unsigned)xcd_grid(P.nbl
 
 template <int F, int NV>
 void launch_finish(SolverState *st, const double *partials, int64_t nblk, hipStream_t s, double *out) {
-    hipLaunchKernelGGL((k_finish<F, NV>), dim3(1), dim3(kFinishBlock), 0, s, st, partials, nblk, out);
+    // about one partial per finish thread, at most kFinishBlocks blocks:
+    // small grids finish in a single block without atomics
+    const int64_t want = (nblk + kFinishBlock - 1) / kFinishBlock;
+    const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>(kFinishBlocks, want));
+    hipLaunchKernelGGL((k_finish<F, NV>), dim3(blocks), dim3(kFinishBlock), 0, s, st, partials, nblk, out);
 }
 
 // ---- LINE preconditioner (LEQ_MSOLVE, leqsol_mod.f:286-598) ---------------

@@ -15,7 +15,8 @@
 ! Input file (stream): 12 int32 (op, imax, jmax, kmax, no_k, cyc_x, cyc_y,
 ! cyc_z, is_serial, minimize_dotproducts, icheck_bicgs, itmax), int32 seed,
 ! real64 tol, 4+4 chars pc/cmethod, then A(N3*7), B(N3), X(N3) real64.
-! Output file (stream): int32 ier, int32 iter_tot, real64 dot results(2),
+! Output file (stream): int32 ier, int32 iter_tot, real64 dot results(2)
+! (op 1: dres(2) = wall seconds of the LEQ_BICGS call),
 ! X(N3), A(N3*7), B(N3), noise(N3).
 program ref_harness
   use param, only: dimension_3
@@ -36,7 +37,8 @@ program ref_harness
   integer :: nseed, ier, ni, nj, nk, i, j, k, ijk, n3, ncls
   integer :: im, ip, jm, jp, km, kp, key, cls
   integer, allocatable :: classof(:)
-  double precision :: dres(2)
+  double precision :: dres(2), elapsed
+  integer(8) :: t0, t1, rate
 
   call get_command_argument(1, fin)
   call get_command_argument(2, fout)
@@ -165,7 +167,10 @@ program ref_harness
      end do
      call random_seed(put=seedarr)
      call random_seed(get=saved)
+     call system_clock(t0, rate)
      call leq_bicgs('HARNESS', 1, xvec, amat, bvec, cmethod, tol, pc, itmax, ier)
+     call system_clock(t1)
+     dres(2) = dble(t1 - t0) / dble(rate)
      call random_seed(put=saved)
      call random_number(noise)
      yvec = xvec
