@@ -103,9 +103,9 @@ int mfix_leq_bicgs_dev(const char *cmethod, double tol, const char *pc, int itma
                        double *d_b_m, const mfix_leq_opts *opts, mfix_leq_stats *stats, void *stream) {
     hipStream_t s;
     int rc;
-    if ((rc = ready(&s, stream))) return rc;
     const int p = parse_pc(pc);
     if (p < 0) return MFIX_LEQ_EBADPC;
+    if ((rc = ready(&s, stream))) return rc;
     if (!d_var || !d_a_m || !d_b_m) return MFIX_LEQ_EINVAL;
     const Context &C = ctx();
     return solve_fp64(make_dgrid(C.grid), p, cmethod ? cmethod : "RSRS", tol, itmax, d_var, d_a_m, d_b_m, opts, stats,
@@ -117,9 +117,9 @@ int mfix_leq_bicgs_mixed_dev(const char *cmethod, double tol, const char *pc, in
                              void *stream) {
     hipStream_t s;
     int rc;
-    if ((rc = ready(&s, stream))) return rc;
     const int p = parse_pc(pc);
     if (p < 0) return MFIX_LEQ_EBADPC;
+    if ((rc = ready(&s, stream))) return rc;
     if (!d_var || !d_a_m || !d_b_m) return MFIX_LEQ_EINVAL;
     return solve_mixed(make_dgrid(ctx().grid), p, cmethod ? cmethod : "RSRS", tol, itmax, d_var, d_a_m, d_b_m, opts,
                        stats, s);
@@ -133,9 +133,9 @@ int mfix_leq_bicgs(const char *vname, int vname_len, int vno, double *var, doubl
     Context &C = ctx();
     hipStream_t s;
     int rc;
-    if ((rc = ready(&s, nullptr))) return rc;
     const int p = parse_pc(pc);
     if (p < 0) return MFIX_LEQ_EBADPC;
+    if ((rc = ready(&s, nullptr))) return rc;
     if (!var || !a_m || !b_m || !ier) return MFIX_LEQ_EINVAL;
     char cm[5] = "    ";
     for (int q = 0; q < 4; ++q) cm[q] = (cmethod && q < cmethod_len) ? cmethod[q] : ' ';
@@ -212,9 +212,9 @@ int mfix_leq_msolve_dev(const char *pc, const char *cmethod, const double *d_a_m
                         void *stream) {
     hipStream_t s;
     int rc;
-    if ((rc = ready(&s, stream))) return rc;
     const int p = parse_pc(pc);
     if (p < 0) return MFIX_LEQ_EBADPC;
+    if ((rc = ready(&s, stream))) return rc;
     if (!d_a_m || !d_r || !d_z) return MFIX_LEQ_EINVAL;
     Context &C = ctx();
     const DGrid g = make_dgrid(C.grid);

@@ -89,6 +89,9 @@ program ref_harness
   allocate(ijk_array_of(istart3:iend3, jstart3:jend3, kstart3:kend3))
   allocate(funijk_map_c(istart3:iend3, jstart3:jend3, kstart3:kend3))
   allocate(imap_c(imin3:imax3), jmap_c(jmin3:jmax3), kmap_c(kmin3:kmax3))
+  ! no cut cells: every DEAD_CELL_AT flag is false (gridmap_mod.f:572-579)
+  allocate(dead_cell_at(imin3-1:imax3+1, jmin3-1:jmax3+1, kmin3-1:kmax3+1))
+  dead_cell_at = .false.
   do i = imin3, imax3; imap_c(i) = i; end do
   do j = jmin3, jmax3; jmap_c(j) = j; end do
   do k = kmin3, kmax3; kmap_c(k) = k; end do
