@@ -198,7 +198,9 @@ int mfix_leq_server_create(size_t hbm_budget_bytes, int max_slots,
 int64_t mfix_leq_server_submit(mfix_leq_server *s, const mfix_leq_request *r);
 /* Advance the device by one scheduling round (admit, iterate, retire). */
 int mfix_leq_server_step(mfix_leq_server *s);
-/* Query a request: returns 1 when done (stats filled, var written), 0 if pending. */
+/* Query a request: returns 1 when done (stats filled, var written), 0 if
+ * pending, or the negative code it failed with.  stats->device_ms is the
+ * wall time from admission to retirement. */
 int mfix_leq_server_poll(mfix_leq_server *s, int64_t id, mfix_leq_stats *stats);
 int mfix_leq_server_pending(mfix_leq_server *s);
 int mfix_leq_server_destroy(mfix_leq_server *s);
@@ -223,6 +225,10 @@ int mfix_leq_set_noise(const double *host_noise, uint64_t seed);
 int mfix_leq_set_write_back(int on);
 /* Statistics of the last mfix_leq_bicgs call. */
 int mfix_leq_last_stats(mfix_leq_stats *st);
+/* 1 (default): iterations of single and batched solves run as replays of a
+ * captured HIP graph (chunks of 16); 0: direct launches.  Results are
+ * bit-identical either way. */
+int mfix_leq_set_graphs(int on);
 
 /* ---- timing of the dominant kernel (HIP events on the solver stream) ---- */
 /* Enable per-launch event timing of kernel `which` (0 = fused stencil

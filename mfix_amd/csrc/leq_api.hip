@@ -62,7 +62,9 @@ int mfix_leq_finalize(void) {
     (void)hipStreamSynchronize(C.stream);
     C.release_all();
     if (C.host_state) (void)hipHostFree(C.host_state);
+    if (C.host_count) (void)hipHostFree(C.host_count);
     C.host_state = nullptr;
+    C.host_count = nullptr;
     (void)hipStreamDestroy(C.stream);
     C.stream = nullptr;
     C.initialised = false;
@@ -228,13 +230,13 @@ int mfix_leq_msolve_dev(const char *pc, const char *cmethod, const double *d_a_m
         run_diag(g, d_a_m, d_r, d_z, s);
         return sync_status(s, "msolve");
     }
-    if ((rc = C.line_scratch.ensure(sizeof(double) * g.n3 * 4))) return rc;
+    if ((rc = C.tmp.ensure(sizeof(double) * g.n3 * 4))) return rc;
     if ((rc = C.misc.ensure(sizeof(int) * 4 + sizeof(double) * 8))) return rc;
     int *flag = (int *)C.misc.ptr;
     e = hipMemsetAsync(flag, 0, sizeof(int), s);
     if (e != hipSuccess) return hip_fail(e, "memset");
     launch_msolve_line<double>(g, C.shape_for(g.n3, false), cmethod ? cmethod : "RSRS", d_a_m, d_r, d_z,
-                               (double *)C.line_scratch.ptr, nullptr, 0, flag, s);
+                               (double *)C.tmp.ptr, nullptr, 0, flag, s, true);
     int hflag = 0;
     e = hipMemcpyAsync(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost, s);
     if (e != hipSuccess) return hip_fail(e, "copy");
@@ -266,6 +268,22 @@ int mfix_leq_scale_dev(double *d_a_m, double *d_b_m, int *ier, void *stream) {
     if ((rc = run_scale(make_dgrid(ctx().grid), d_a_m, d_b_m, &zero, s))) return rc;
     *ier = zero ? -2 : 0;
     return 0;
+}
+
+int mfix_leq_batch_solve(int nsys, const mfix_leq_batch_item *items, mfix_leq_stats *stats, void *stream) {
+    if (nsys < 0 || (nsys > 0 && !items)) return MFIX_LEQ_EINVAL;
+    if (nsys > kHostStates) return MFIX_LEQ_EINVAL;
+    for (int q = 0; q < nsys; ++q) {
+        const mfix_leq_batch_item &it = items[q];
+        if (parse_pc(it.pc) < 0) return MFIX_LEQ_EBADPC;
+        if (it.imax < 1 || it.jmax < 1 || it.kmax < 1 || !it.d_var || !it.d_a_m || !it.d_b_m)
+            return MFIX_LEQ_EINVAL;
+        if ((double)(it.imax + 2) * (it.jmax + 2) * (it.kmax + 2) >= 2147483647.0) return MFIX_LEQ_EINVAL;
+    }
+    int rc;
+    if ((rc = ctx().ensure_init())) return rc;
+    if (nsys == 0) return 0;
+    return solve_batch(nsys, items, stats, stream ? (hipStream_t)stream : ctx().stream);
 }
 
 int mfix_leq_autotune(int imax, int jmax, int kmax, int mixed, int *block, int *lines_per_block) {
@@ -305,6 +323,11 @@ int mfix_leq_set_noise(const double *host_noise, uint64_t seed) {
 
 int mfix_leq_set_write_back(int on) {
     ctx().write_back_scaled = on != 0;
+    return 0;
+}
+
+int mfix_leq_set_graphs(int on) {
+    ctx().use_graphs = on != 0;
     return 0;
 }
 

@@ -1,15 +1,17 @@
 // Host-side state of the solver library: one device context per process
 // (device, stream, grid descriptor, grow-only workspaces, pinned state
-// mirror, per-kernel event timing, tuned launch shapes).
+// mirror, per-kernel event timing, tuned launch shapes) and the system
+// tables that every solve (single, batched, served) runs through.
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <functional>
 #include <utility>
 #include <vector>
 
 #include "../../include/mfix_leq.h"
-#include "leq_kernels.h"
+#include "leq_phases.h"
 
 namespace mfix {
 
@@ -34,9 +36,10 @@ struct Prof {
     void collect();
 };
 constexpr int kNumProf = 2;
-constexpr int kHostStates = 1024;
+constexpr int kHostStates = 1024; // pinned state mirrors (max systems read back at once)
 constexpr int kMaxPartials = 4;   // partial-sum arrays any reduction kernel writes (NV <= 4)
 constexpr int kSizeClasses = 12;  // 8K, 32K, ..., 4^11 * 8K cells
+constexpr int kMaxRPasses = 4;    // 'R' characters of a 4-character LINE sweep string
 
 // Launch shape of the streaming kernels: threads per block, cells per
 // thread, and threads (= lines) per block of the line-solve kernels.
@@ -46,6 +49,42 @@ struct Shape {
 
 int size_class(int64_t n3);
 
+using Sweep = std::array<char, 4>;
+
+// Block maps of one launch sequence over the systems `ids` of a table:
+// stream kernels (smap), finish kernels (fmap), batched red-black line
+// phases (lmap[pass]); LINE systems whose sweep string is not made of R/S
+// passes are swept one by one (line_seq).
+struct MapSet {
+    std::vector<int> ids;
+    std::vector<int2> smap, fmap, lmap[kMaxRPasses];
+    int nlpass = 0;
+    std::vector<int> line_seq;
+    double cells = 0.0;           // sum of N3 over ids (timing bytes)
+    DevBuf dev;
+    int *d_ids = nullptr;
+    int2 *d_smap = nullptr, *d_fmap = nullptr, *d_lmap[kMaxRPasses] = {};
+    uint64_t version = 0;         // bumped whenever the device copy changes
+};
+
+// A table of systems of one storage type and its device copy.
+template <typename T>
+struct Group {
+    std::vector<Sys<T>> sys;
+    std::vector<Sweep> sweep;     // LINE sweep string of each system
+    int bs = 256;                 // threads per block of the stream kernels
+    bool mixed = false;
+    DevBuf dev;
+    Sys<T> *d_sys = nullptr;
+    MapSet iter;                  // systems being iterated
+    // captured chunk of iterations (replayed until every system is done)
+    hipGraphExec_t exec = nullptr;
+    int exec_iters = 0;
+    uint64_t exec_version = 0;
+    const void *exec_sys = nullptr;
+    void drop_graph();
+};
+
 struct Context {
     bool initialised = false;
     bool have_grid = false;
@@ -54,12 +93,18 @@ struct Context {
     mfix_leq_grid grid = {};
     int iter_tot[10] = {0};
     DevBuf d_a, d_b, d_x;          // host-entry staging of A_M, B_M, VAR
-    DevBuf ws64, ws32, a32, xalt;  // Krylov workspaces
-    DevBuf line_scratch;           // LINE preconditioner line scratch (4 planes)
-    DevBuf partials, state, misc;
-    SolverState *host_state = nullptr;  // pinned mirror(s) of device states
+    DevBuf ws;                     // single-solve workspace (carved per solve)
+    DevBuf bws;                    // batch workspace
+    DevBuf partials, state, misc, tmp;
+    SolverState *host_state = nullptr;  // pinned mirrors of device states
+    int *host_count = nullptr;          // pinned running-system count
+    Group<double> g64;             // single-solve tables
+    Group<float> g32;
+    Group<double> b64;             // batch tables
+    Group<float> b32;
     Prof prof[kNumProf];
     Shape tuned[2][kSizeClasses];  // [fp32][size class]; bs == 0 means untuned
+    int use_graphs = 1;            // capture chunks of iterations in HIP graphs
     hipError_t last_hip_error = hipSuccess;
     char last_msg[256] = {0};
     const double *host_noise = nullptr;
@@ -78,34 +123,48 @@ int parse_pc(const char *pc);
 DGrid make_dgrid(const mfix_leq_grid &h);
 double alg_bytes_per_cell(int pc, int sa, int sv, int sx);
 
-// Pointers of one solve's vectors (T = double or float).  x is VAR (FP64) or
-// the FP32 correction accumulator (mixed); phat/shat exist only for LINE.
-template <typename T>
-struct SolveBufs {
+// Description of one system to solve (device arrays are the caller's).
+struct SysDesc {
     DGrid g;
-    const T *a = nullptr;
-    T *x = nullptr, *r = nullptr, *rt = nullptr, *p = nullptr, *phat = nullptr;
-    T *s = nullptr, *shat = nullptr, *t = nullptr, *v = nullptr;
-    T *line_scratch = nullptr;
-    SolverState *st = nullptr;
-    double *partials = nullptr;
-    bool mixed = false;
+    int pc;
+    Sweep sweep;
+    double tol;
+    int itmax;
+    int minimize, icheck;
+    double delta;                 // mixed: reliable-update trigger
+    double *x, *a, *b;            // VAR, A_M, B_M (FP64, device)
+    const double *noise;
+    uint64_t seed;
 };
 
-template <int F, int NV>
-void launch_finish(SolverState *st, const double *partials, int64_t nblk, hipStream_t s, double *out);
+// Workspace bytes of one system and the carving of that workspace.
+size_t sys_bytes(const DGrid &g, const Part &P, int pc, bool mixed);
 template <typename T>
-int launch_msolve_line(const DGrid &g, const Shape &sh, const char *cmethod, const T *a, const T *r, T *z,
-                       T *scratch, const SolverState *st, int skip_on_sexit, int *fatal_flag, hipStream_t s);
+void sys_carve(Sys<T> &S, const SysDesc &d, const Part &P, char *base);
+
 template <typename T>
-void enqueue_iteration(const SolveBufs<T> &w, const Shape &sh, int pc, const char *cmethod, hipStream_t s);
-int drive(const std::function<void()> &one_iteration, SolverState *d_state, SolverState *h_state, int itmax,
-          hipStream_t s);
+int build_maps(const Group<T> &G, const std::vector<int> &ids, MapSet &M, hipStream_t s);
+template <typename T>
+int upload_table(Group<T> &G, hipStream_t s);
+template <typename T>
+void enqueue_setup(Group<T> &G, const MapSet &M, hipStream_t s);
+template <typename T>
+void enqueue_iterations(Group<T> &G, const MapSet &M, int n, hipStream_t s);
+template <typename T>
+int run_group(Group<T> &G, int max_iters, hipStream_t s, bool graphs);
+template <typename T>
+int read_states(const Group<T> &G, const std::vector<int> &ids, SolverState *h, hipStream_t s);
+
 void fill_stats(const SolverState &h, mfix_leq_stats *st);
 int solve_fp64(const DGrid &g, int pc, const char *cmethod, double tol, int itmax, double *x, double *a, double *b,
                const mfix_leq_opts *opts, mfix_leq_stats *stats, hipStream_t s, int minimize, int icheck);
 int solve_mixed(const DGrid &g, int pc, const char *cmethod, double tol, int itmax, double *x, double *a, double *b,
                 const mfix_leq_opts *opts, mfix_leq_stats *stats, hipStream_t s);
+int solve_batch(int nsys, const mfix_leq_batch_item *items, mfix_leq_stats *stats, hipStream_t s);
+template <typename T>
+int launch_msolve_line(const DGrid &g, const Shape &sh, const char *cmethod, const T *a, const T *r, T *z,
+                       T *scratch, const SolverState *st, int skip_on_sexit, int *fatal_flag, hipStream_t s,
+                       bool copy_first);
 int run_matvec64(const DGrid &g, const double *a, const double *x, double *y, hipStream_t s);
 int run_matvec32(const DGrid &g, const float *a, const float *x, float *y, hipStream_t s);
 int run_diag(const DGrid &g, const double *a, const double *r, double *z, hipStream_t s);

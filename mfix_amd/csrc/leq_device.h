@@ -264,8 +264,9 @@ __device__ __forceinline__ void block_partial(double (&v)[NV], double *partials,
     }
 }
 
-// Finish step, run by kFinishBlocks blocks of kFinishBlock threads.  Block
-// b sums the partials of slice b in a fixed order into level-2 sums; the
+// Finish step, run by nfb <= kFinishBlocks blocks of kFinishBlock threads
+// (fb = this block's index among them).  Block fb sums the partials of
+// slice fb in a fixed order into level-2 sums; the
 // last block to arrive (counted with one device-scope atomic per block)
 // adds the level-2 sums in slice order, so the result does not depend on
 // arrival order.  Only the few level-2 values are written before the
@@ -279,11 +280,11 @@ __device__ __forceinline__ double load_agent(const double *p) {
 }
 
 template <int NV>
-__device__ __forceinline__ bool finish_sum(const double *partials, int64_t nblk, double *l2, unsigned *counter,
-                                           double *lds, double (&tot)[NV]) {
+__device__ __forceinline__ bool finish_sum(const double *partials, int64_t nblk, int fb, int nfb, double *l2,
+                                           unsigned *counter, double *lds, double (&tot)[NV]) {
     __shared__ bool last;
-    const int64_t per = (nblk + gridDim.x - 1) / gridDim.x;
-    const int64_t lo = (int64_t)blockIdx.x * per;
+    const int64_t per = (nblk + nfb - 1) / nfb;
+    const int64_t lo = (int64_t)fb * per;
     const int64_t hi = lo + per < nblk ? lo + per : nblk;
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
@@ -293,20 +294,19 @@ __device__ __forceinline__ bool finish_sum(const double *partials, int64_t nblk,
         tot[q] = s;
     }
     block_sum<kFinishBlock, NV>(tot, lds);
-    if (gridDim.x == 1) return true;
+    if (nfb == 1) return true;
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int q = 0; q < NV; ++q) l2[q * gridDim.x + blockIdx.x] = tot[q];
+        for (int q = 0; q < NV; ++q) l2[q * nfb + fb] = tot[q];
         __threadfence();
         const unsigned prev = atomicAdd(counter, 1u);
-        last = prev == gridDim.x - 1;
+        last = prev == (unsigned)nfb - 1;
     }
     __syncthreads();
     if (!last) return false;
     __threadfence();
 #pragma unroll
-    for (int q = 0; q < NV; ++q)
-        tot[q] = threadIdx.x < gridDim.x ? load_agent(&l2[q * gridDim.x + threadIdx.x]) : 0.0;
+    for (int q = 0; q < NV; ++q) tot[q] = threadIdx.x < nfb ? load_agent(&l2[q * nfb + threadIdx.x]) : 0.0;
     __syncthreads();
     block_sum<kFinishBlock, NV>(tot, lds);
     if (threadIdx.x == 0) *counter = 0u;

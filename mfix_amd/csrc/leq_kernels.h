@@ -281,16 +281,6 @@ __device__ __forceinline__ void fin_apply(SolverState *st, const double (&tot)[N
     }
 }
 
-template <int F, int NV>
-__global__ void __launch_bounds__(kFinishBlock) k_finish(SolverState *st, const double *__restrict__ partials,
-                                                         int64_t nblk, double *out) {
-    __shared__ double lds[(kFinishBlock / 64) * 4];
-    if (fin_skips<F>(st)) return;
-    double tot[NV];
-    if (finish_sum<NV>(partials, nblk, st->l2, &st->fin_counter, lds, tot) && threadIdx.x == 0)
-        fin_apply<F, NV>(st, tot, out);
-}
-
 // ---- LINE preconditioner (LEQ_MSOLVE, leqsol_mod.f:286-598) ---------------
 //
 // One thread solves one line with the DGTSV algorithm of model/DGTSV.f:107-163
@@ -475,6 +465,46 @@ __global__ void __launch_bounds__(64) k_line_phase(DGrid g, LinePhase ph, const 
     }
     const int64_t c0 = (int64_t)u * L.su + (int64_t)v * L.sv;
     solve_line<T>(g, L, ph.axis, c0, a, bm, var, sdl, sd, sdu, sb, ph.fatal, fatal_flag);
+}
+
+// Lines of one red-black colour along J, numbered densely: line t of the
+// colour -> (u, v) = (i-1, k-1).  The colour of line (u, v) is the parity of
+// (u+1) + (v+1)*vw (vw = 1 in 3-D, leqsol_mod.f:497-522; vw = 0 in 2-D,
+// where only I alternates, :367-392).  Row v holds the u = u0(v), u0+2, ...
+// In 3-D two consecutive rows hold nu lines together.
+__host__ __device__ inline bool color_line(int nu, int nv, int vw, int param, int64_t t, int &u, int &v) {
+    if (vw == 0) {
+        const int u0 = (param + 1) & 1;
+        const int cnt = (nu - u0 + 1) / 2;
+        if (cnt <= 0) return false;
+        v = (int)(t / cnt);
+        if (v >= nv) return false;
+        u = u0 + 2 * (int)(t % cnt);
+        return true;
+    }
+    const int64_t m = t / nu;
+    const int r = (int)(t - m * nu);
+    v = (int)(2 * m);
+    const int u0 = (param + v) & 1;
+    const int cnt0 = (nu - u0 + 1) / 2;
+    if (r < cnt0) {
+        u = u0 + 2 * r;
+    } else {
+        v += 1;
+        u = ((param + v) & 1) + 2 * (r - cnt0);
+    }
+    return v < nv;
+}
+
+// Number of lines of a colour (an upper bound over both colours is
+// (nu*nv+1)/2).
+__host__ __device__ inline int64_t color_count(int nu, int nv, int vw, int param) {
+    int64_t n = 0;
+    for (int v = 0; v < nv; ++v) {
+        const int u0 = vw ? ((param + v) & 1) : ((param + 1) & 1);
+        n += (nu - u0 + 1) / 2;
+    }
+    return n;
 }
 
 }  // namespace mfix

@@ -1,52 +1,92 @@
 // Streaming phase kernels of one BiCGSTAB iteration (LEQ_BICGS0,
-// model/leq_bicgs.f:316-466) and its set-up (:194-305).
+// model/leq_bicgs.f:316-466) and its set-up (:194-305), for one or many
+// independent systems per launch.
 //
-// Each kernel is a pass over the cells with BS threads per block; a
-// reduction phase writes one partial per block and is followed by a
-// k_finish launch (leq_kernels.h).  With the DIAG preconditioner
-// (LEQ_MSOLVE1, leqsol_mod.f:729-744) Phat = P/A0 and Shat = S/A0 are never
-// stored: the stencil kernels divide each operand by its A0 as they load it
-// and the solution update divides again, which rounds exactly like the
-// reference's separate division pass and saves two vector writes and reads
-// per preconditioner application.
+// A launch covers a table of systems (Sys<T>) through a block map: hardware
+// block b works on system map[b].x, logical block map[b].y (or returns at
+// once for a padding entry).  Each system owns a contiguous, 8-aligned range
+// of hardware blocks whose logical blocks are permuted so that XCD x (blocks
+// b % 8 == x) holds one contiguous slab of that system's cells.  Every
+// system keeps its own SolverState and partial sums and its blocks return as
+// soon as it is done, so converged systems cost nothing in later launches
+// and a solve gives the same bits alone or in a batch.  A single solve is a
+// table of one.
+//
+// With the DIAG preconditioner (LEQ_MSOLVE1, leqsol_mod.f:729-744) Phat =
+// P/A0 and Shat = S/A0 are never stored: the stencil kernels divide each
+// operand by its A0 as they load it and the solution update divides again,
+// which rounds exactly like the reference's separate division pass and saves
+// two vector writes and reads per preconditioner application.
 #pragma once
 #include "leq_kernels.h"
 
 namespace mfix {
 
-// Fresh solver state: the whole struct is rewritten, so nothing of a previous
-// solve carries over.
-__global__ void k_init_state(SolverState *st, double tol, int itmax, int minimize, int icheck, int mixed,
-                             double delta) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        SolverState s = {};
-        s.tol = tol;
-        s.itmax = itmax;
-        s.i = 1;
-        s.iter_var = 1;
-        s.minimize = minimize;
-        s.icheck = icheck > 0 ? icheck : 1;
-        s.mixed = mixed;
-        s.delta = delta;
-        s.true_rnorm = -1.0;
-        *st = s;
-    }
+// One system of a launch.  T is the Krylov storage type (double: plain FP64
+// solve; float: inner solve of the mixed path).  The FP64 arrays (matrix,
+// rhs, solution buffers, true residual) sit beside the T ones; for a plain
+// FP64 solve a64 aliases a, b64 is B_M, xa is VAR and r64 is r.
+template <typename T>
+struct Sys {
+    DGrid g;
+    Part P;                // cell split: P.nblk logical blocks of P.chunk cells
+    int nfin;              // blocks of this system in finish launches
+    int pc;                // PC_NONE / PC_DIAG / PC_LINE
+    int itmax, minimize, icheck, mixed;
+    double tol, delta;
+    const T *a;            // reference layout (FP64) or difference layout (FP32)
+    T *x, *r, *rt, *p, *s, *t, *v, *phat, *shat;
+    T *line_scratch;       // 4 planes of DGTSV scratch (LINE)
+    SolverState *st;
+    double *partials;      // kMaxPartials * P.nblk
+    double *a64, *b64, *xa, *xb, *r64;
+    const double *noise;   // optional U[0,1) array for Rtilde
+    uint64_t seed;
+};
+
+// Map entry -> (system, logical block); returns from the kernel for padding.
+#define SYS_BLOCK(T)                             \
+    const int2 e_ = map[blockIdx.x];             \
+    if (e_.x < 0) return;                        \
+    const Sys<T> &S = sys[e_.x];                 \
+    const int64_t blk = e_.y;                    \
+    const DGrid &g = S.g;                        \
+    const Part &P = S.P;
+
+// Fresh solver state of the listed systems (the whole struct is rewritten,
+// so nothing of a previous solve carries over).
+template <typename T>
+__global__ void k_init_states(const Sys<T> *__restrict__ sys, const int *__restrict__ ids, int nids) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nids) return;
+    const Sys<T> &S = sys[ids[q]];
+    SolverState s = {};
+    s.tol = S.tol;
+    s.itmax = S.itmax;
+    s.i = 1;
+    s.iter_var = 1;
+    s.minimize = S.minimize;
+    s.icheck = S.icheck > 0 ? S.icheck : 1;
+    s.mixed = S.mixed;
+    s.delta = S.delta;
+    s.true_rnorm = -1.0;
+    *S.st = s;
 }
 
 // Unit scaling (leq_bicgs.f:194-233) fused with R = B - A*Var (:283-287) and
 // the partial sums of <R,R> (:291).  Each row is scaled by its own max-norm,
-// so the residual row uses only this thread's coefficients.
-template <int BS>
-__global__ void __launch_bounds__(BS) k_setup_residual(DGrid g, Part P, double *__restrict__ a,
-                                                       double *__restrict__ b, const double *__restrict__ x,
-                                                       double *__restrict__ r, SolverState *st, double *partials) {
+// so the residual row uses only this thread's coefficients.  FP64 arrays.
+template <int BS, typename T>
+__global__ void __launch_bounds__(BS) k_setup_residual(const Sys<T> *__restrict__ sys, const int2 *__restrict__ map) {
     __shared__ double lds[(BS / 64) * 1];
-    if (!xcd_valid(P.nblk)) return;
-    const int64_t blk = xcd_block(P.nblk);
+    SYS_BLOCK(T)
+    double *__restrict__ a = S.a64;
+    double *__restrict__ b = S.b64;
+    double *__restrict__ r = S.r64;
     double v[1] = {0.0};
     const int64_t n3 = g.n3;
     int zero = 0;
-    const Operand<double, false> X{x, nullptr};
+    const Operand<double, false> X{S.xa, nullptr};
     FOR_CELLS(BS, P, blk, c) {
         double amax = 0.0;
 #pragma unroll
@@ -64,136 +104,155 @@ __global__ void __launch_bounds__(BS) k_setup_residual(DGrid g, Part P, double *
         r[c] = rr;
         if (g.dot_all || n.interior) v[0] += rr * rr;
     }
-    if (zero) atomicOr(&st->zero_row, 1);
-    block_partial<BS, 1>(v, partials, P.nblk, blk, lds);
+    if (zero) atomicOr(&S.st->zero_row, 1);
+    block_partial<BS, 1>(v, S.partials, P.nblk, blk, lds);
 }
 
 // Rtilde = R + (2U-1)*1e-6*Rnorm0 (leq_bicgs.f:298-305) and rho_0 = <Rtilde,R>
 // (:318).  For the mixed path (T = float) R and Rtilde are rounded from the
 // FP64 residual and the FP32 correction accumulator is cleared.
 template <int BS, typename T>
-__global__ void __launch_bounds__(BS) k_shadow(DGrid g, Part P, const double *__restrict__ r64, T *__restrict__ r,
-                                               T *__restrict__ rt, T *__restrict__ y,
-                                               const double *__restrict__ noise, uint64_t seed,
-                                               const SolverState *st, double *partials) {
+__global__ void __launch_bounds__(BS) k_shadow(const Sys<T> *__restrict__ sys, const int2 *__restrict__ map) {
     __shared__ double lds[(BS / 64) * 1];
-    if (st->done) return;
-    if (!xcd_valid(P.nblk)) return;
-    const int64_t blk = xcd_block(P.nblk);
+    SYS_BLOCK(T)
+    if (S.st->done) return;
     double v[1] = {0.0};
-    const double rn0 = st->rnorm0;
+    const double rn0 = S.st->rnorm0;
+    const double *__restrict__ r64 = S.r64;
+    T *__restrict__ rt = S.rt;
+    const bool mixed = S.mixed;
     FOR_CELLS(BS, P, blk, c) {
-        const double u = noise ? noise[c] : urand(seed, 100, (uint64_t)c);
+        const double u = S.noise ? S.noise[c] : urand(S.seed, 100, (uint64_t)c);
         const double rc = r64[c];
         const T t = (T)(rc + (2.0 * u - 1.0) * 1.0e-6 * rn0);
         rt[c] = t;
         const T rT = (T)rc;
-        if (y) {
-            r[c] = rT;
-            y[c] = (T)0;
+        if (mixed) {
+            S.r[c] = rT;
+            S.x[c] = (T)0;
         }
         if (g.dot_all || is_interior(g, c)) v[0] += (double)t * (double)rT;
     }
-    block_partial<BS, 1>(v, partials, P.nblk, blk, lds);
+    block_partial<BS, 1>(v, S.partials, P.nblk, blk, lds);
 }
 
 // P = R (i=1, or after a restart) or P = R + beta*(P - omega*V)
-// (leq_bicgs.f:334-343).  A restart also resets Rtilde = R.
+// (leq_bicgs.f:334-343).  A restart also resets Rtilde = R.  For LINE the
+// preconditioner's start value Phat = P (z = r, leqsol_mod.f:341-357) is
+// written in the same pass.
 template <int BS, typename T>
-__global__ void __launch_bounds__(BS) k_p_update(DGrid g, Part P, const T *__restrict__ r, T *__restrict__ p,
-                                                 const T *__restrict__ v, T *__restrict__ rt, const SolverState *st) {
+__global__ void __launch_bounds__(BS) k_p_update(const Sys<T> *__restrict__ sys, const int2 *__restrict__ map) {
+    SYS_BLOCK(T)
+    const SolverState *st = S.st;
     if (st->done) return;
-    if (!xcd_valid(P.nblk)) return;
-    const int64_t blk = xcd_block(P.nblk);
     const bool restart = st->restart;
     const bool first = st->i == 1 || restart;
     const double beta = st->beta, omega = st->omega;
+    const T *__restrict__ r = S.r;
+    T *__restrict__ p = S.p;
+    const T *__restrict__ v = S.v;
+    T *__restrict__ ph = S.pc == PC_LINE ? S.phat : nullptr;
     FOR_CELLS(BS, P, blk, c) {
         const T rc = r[c];
-        if (restart) rt[c] = rc;
+        if (restart) S.rt[c] = rc;
         double pv;
         if (first) pv = (double)rc;
         else pv = (double)rc + beta * ((double)p[c] - omega * (double)v[c]);
         p[c] = (T)pv;
+        if (ph) ph[c] = (T)pv;
     }
 }
 
 // MODE 0: V = A*Phat (leq_bicgs.f:355) and <Rtilde,V> (:357).
 // MODE 1: T = A*Shat (:409), <T,S>, <T,T> (:411-421).
-// DIV: the operand is x/A0 (DIAG), x otherwise (NONE: x = P or S; LINE:
-// x = the preconditioned array).
-template <int BS, typename T, int MODE, bool DIV>
-__global__ void __launch_bounds__(BS) k_matvec_dots(DGrid g, Part P, const T *__restrict__ a,
-                                                    const T *__restrict__ x, T *__restrict__ y,
-                                                    const T *__restrict__ w, const SolverState *st,
-                                                    double *partials) {
+// The operand is P/A0 or S/A0 for DIAG, the preconditioned array for LINE,
+// and P or S for NONE.
+template <int BS, typename T, int MODE>
+__global__ void __launch_bounds__(BS) k_matvec_dots(const Sys<T> *__restrict__ sys, const int2 *__restrict__ map) {
     __shared__ double lds[(BS / 64) * 2];
+    SYS_BLOCK(T)
+    const SolverState *st = S.st;
     if (st->done) return;
     if (MODE == 1 && st->s_exit) return;
-    if (!xcd_valid(P.nblk)) return;
-    const int64_t blk = xcd_block(P.nblk);
     double v[2] = {0.0, 0.0};
     const bool all = MODE == 1 ? (g.dot_all && !st->minimize) : g.dot_all;
-    const Operand<T, DIV> X{x, a + (int64_t)DiagPlane<T>::value * g.n3};
-    FOR_CELLS(BS, P, blk, c) {
-        const Nb n = neighbours_fast(g, c);
-        const T yy = Stencil<T>::row(g, a, X, c, n);
-        y[c] = yy;
-        if (all || n.interior) {
-            const double yd = (double)yy, wd = (double)w[c];
-            v[0] += (MODE == 0) ? wd * yd : yd * wd;
-            if (MODE == 1) v[1] += yd * yd;
+    const T *__restrict__ a = S.a;
+    const T *__restrict__ a0 = a + (int64_t)DiagPlane<T>::value * g.n3;
+    T *__restrict__ y = MODE == 0 ? S.v : S.t;
+    const T *__restrict__ w = MODE == 0 ? S.rt : S.s;
+    const T *__restrict__ xin = MODE == 0 ? (S.pc == PC_LINE ? S.phat : S.p) : (S.pc == PC_LINE ? S.shat : S.s);
+    auto body = [&](auto X) {
+        FOR_CELLS(BS, P, blk, c) {
+            const Nb n = neighbours_fast(g, c);
+            const T yy = Stencil<T>::row(g, a, X, c, n);
+            y[c] = yy;
+            if (all || n.interior) {
+                const double yd = (double)yy, wd = (double)w[c];
+                v[0] += (MODE == 0) ? wd * yd : yd * wd;
+                if (MODE == 1) v[1] += yd * yd;
+            }
         }
-    }
+    };
+    if (S.pc == PC_DIAG) body(Operand<T, true>{xin, a0});
+    else body(Operand<T, false>{xin, nullptr});
     if (MODE == 0) {
         double v1[1] = {v[0]};
-        block_partial<BS, 1>(v1, partials, P.nblk, blk, lds);
+        block_partial<BS, 1>(v1, S.partials, P.nblk, blk, lds);
     } else {
-        block_partial<BS, 2>(v, partials, P.nblk, blk, lds);
+        block_partial<BS, 2>(v, S.partials, P.nblk, blk, lds);
     }
 }
 
-// S = R - alpha*V (leq_bicgs.f:365-367) and ||S||^2 (:373).
+// S = R - alpha*V (leq_bicgs.f:365-367) and ||S||^2 (:373); for LINE also
+// the preconditioner's start value Shat = S.
 template <int BS, typename T>
-__global__ void __launch_bounds__(BS) k_s_update(DGrid g, Part P, const T *__restrict__ r, const T *__restrict__ v,
-                                                 T *__restrict__ s, const SolverState *st, double *partials) {
+__global__ void __launch_bounds__(BS) k_s_update(const Sys<T> *__restrict__ sys, const int2 *__restrict__ map) {
     __shared__ double lds[(BS / 64) * 1];
+    SYS_BLOCK(T)
+    const SolverState *st = S.st;
     if (st->done) return;
-    if (!xcd_valid(P.nblk)) return;
-    const int64_t blk = xcd_block(P.nblk);
     double acc[1] = {0.0};
     const double alpha = st->alpha;
+    const T *__restrict__ r = S.r;
+    const T *__restrict__ v = S.v;
+    T *__restrict__ s = S.s;
+    T *__restrict__ sh = S.pc == PC_LINE ? S.shat : nullptr;
     FOR_CELLS(BS, P, blk, c) {
         const T sv = (T)((double)r[c] - alpha * (double)v[c]);
         s[c] = sv;
+        if (sh) sh[c] = sv;
         if (g.dot_all || is_interior(g, c)) acc[0] += (double)sv * (double)sv;
     }
-    block_partial<BS, 1>(acc, partials, P.nblk, blk, lds);
+    block_partial<BS, 1>(acc, S.partials, P.nblk, blk, lds);
 }
 
 // Var += alpha*Phat + omega*Shat; R = S - omega*T (leq_bicgs.f:432-436);
 // ||R||^2 (:440) and the next <Rtilde,R> (:318).  After the S-norm exit only
-// Var += alpha*Phat is applied (:374-376).  ph/sh are P/S (DIV: divided by
-// A0 here) or the preconditioned arrays.  In the mixed path x is the FP32
+// Var += alpha*Phat is applied (:374-376).  In the mixed path x is the FP32
 // correction accumulator, restarted at 0 after a reliable update.
-template <int BS, typename T, bool DIV>
-__global__ void __launch_bounds__(BS) k_xr_update(DGrid g, Part P, T *__restrict__ x, const T *__restrict__ ph,
-                                                  const T *__restrict__ sh, const T *__restrict__ a0,
-                                                  const T *__restrict__ s, const T *__restrict__ t,
-                                                  T *__restrict__ r, const T *__restrict__ rt,
-                                                  const SolverState *st, double *partials) {
+template <int BS, typename T>
+__global__ void __launch_bounds__(BS) k_xr_update(const Sys<T> *__restrict__ sys, const int2 *__restrict__ map) {
     __shared__ double lds[(BS / 64) * 2];
+    SYS_BLOCK(T)
+    const SolverState *st = S.st;
     if (st->done) return;
-    if (!xcd_valid(P.nblk)) return;
-    const int64_t blk = xcd_block(P.nblk);
     double acc[2] = {0.0, 0.0};
     const double alpha = st->alpha, omega = st->omega;
     const bool sexit = st->s_exit;
     const bool yr = st->y_reset;
+    const bool div = S.pc == PC_DIAG;
+    const T *__restrict__ ph = S.pc == PC_LINE ? S.phat : S.p;
+    const T *__restrict__ sh = S.pc == PC_LINE ? S.shat : S.s;
+    const T *__restrict__ a0 = S.a + (int64_t)DiagPlane<T>::value * g.n3;
+    T *__restrict__ x = S.x;
+    const T *__restrict__ s = S.s;
+    const T *__restrict__ t = S.t;
+    T *__restrict__ r = S.r;
+    const T *__restrict__ rt = S.rt;
     FOR_CELLS(BS, P, blk, c) {
         const double x0 = yr ? 0.0 : (double)x[c];
         double phv, d0 = 1.0;
-        if (DIV) {
+        if (div) {
             d0 = (double)a0[c];
             phv = (double)(T)((double)ph[c] / d0);
         } else {
@@ -202,7 +261,7 @@ __global__ void __launch_bounds__(BS) k_xr_update(DGrid g, Part P, T *__restrict
         if (sexit) {
             x[c] = (T)(x0 + alpha * phv);
         } else {
-            const double shv = DIV ? (double)(T)((double)sh[c] / d0) : (double)sh[c];
+            const double shv = div ? (double)(T)((double)sh[c] / d0) : (double)sh[c];
             x[c] = (T)(x0 + alpha * phv + omega * shv);
             const T rv = (T)((double)s[c] - omega * (double)t[c]);
             r[c] = rv;
@@ -212,32 +271,32 @@ __global__ void __launch_bounds__(BS) k_xr_update(DGrid g, Part P, T *__restrict
             }
         }
     }
-    block_partial<BS, 2>(acc, partials, P.nblk, blk, lds);
+    block_partial<BS, 2>(acc, S.partials, P.nblk, blk, lds);
 }
 
 // Reliable update (mixed path): xo = xi + y, r64 = b - A64*(xi + y) with the
 // neighbours' xi + y formed on the fly (xi is read-only here, xo is the other
 // buffer), r = (float)r64; partial ||r64||^2, <Rtilde, r> and <r, r>.
 template <int BS>
-__global__ void __launch_bounds__(BS) k_reliable_update(DGrid g, Part P, const double *__restrict__ a,
-                                                        const double *__restrict__ b, const double *__restrict__ x0,
-                                                        const double *__restrict__ x1, double *xo0, double *xo1,
-                                                        const float *__restrict__ y, float *__restrict__ r,
-                                                        const float *__restrict__ rt, const SolverState *st,
-                                                        double *partials) {
+__global__ void __launch_bounds__(BS) k_reliable_update(const Sys<float> *__restrict__ sys,
+                                                        const int2 *__restrict__ map) {
     __shared__ double lds[(BS / 64) * 3];
+    SYS_BLOCK(float)
+    const SolverState *st = S.st;
     if (st->done || !st->need_update) return;
-    if (!xcd_valid(P.nblk)) return;
-    const int64_t blk = xcd_block(P.nblk);
     const int cur = st->x_cur;
-    const double *__restrict__ xi = cur ? x1 : x0;
-    double *__restrict__ xo = cur ? xo0 : xo1;
+    const double *__restrict__ xi = cur ? S.xb : S.xa;
+    double *__restrict__ xo = cur ? S.xa : S.xb;
+    const double *__restrict__ a = S.a64;
+    const double *__restrict__ b = S.b64;
+    float *__restrict__ r = S.r;
+    const float *__restrict__ rt = S.rt;
     double acc[3] = {0.0, 0.0, 0.0};
     struct XY {
         const double *__restrict__ xi;
         const float *__restrict__ y;
         __device__ __forceinline__ double operator()(int64_t q) const { return xi[q] + (double)y[q]; }
-    } X{xi, y};
+    } X{xi, S.x};
     FOR_CELLS(BS, P, blk, c) {
         const Nb n = neighbours_fast(g, c);
         const double rr = b[c] - stencil_row<double>(g, a, X, c, n);
@@ -250,7 +309,7 @@ __global__ void __launch_bounds__(BS) k_reliable_update(DGrid g, Part P, const d
             acc[2] += (double)r32 * (double)r32;
         }
     }
-    block_partial<BS, 3>(acc, partials, P.nblk, blk, lds);
+    block_partial<BS, 3>(acc, S.partials, P.nblk, blk, lds);
 }
 
 // FP32 difference-layout copy of the (scaled) FP64 matrix: off-diagonals
@@ -258,10 +317,11 @@ __global__ void __launch_bounds__(BS) k_reliable_update(DGrid g, Part P, const d
 // A coupling clamped to the cell itself contributes a_l*(x_c - x_c) = 0 in
 // the difference form and a_l*x_c through the excess, as in LEQ_MATVEC.
 template <int BS>
-__global__ void __launch_bounds__(BS) k_prepare_a32(DGrid g, Part P, const double *__restrict__ a,
-                                                    float *__restrict__ a32) {
-    if (!xcd_valid(P.nblk)) return;
-    const int64_t blk = xcd_block(P.nblk);
+__global__ void __launch_bounds__(BS) k_prepare_a32(const Sys<float> *__restrict__ sys, const int2 *__restrict__ map) {
+    SYS_BLOCK(float)
+    if (S.st->done) return;
+    const double *__restrict__ a = S.a64;
+    float *__restrict__ a32 = const_cast<float *>(S.a);
     const int64_t n3 = g.n3;
     FOR_CELLS(BS, P, blk, c) {
         double e = a[3 * n3 + c];
@@ -275,6 +335,57 @@ __global__ void __launch_bounds__(BS) k_prepare_a32(DGrid g, Part P, const doubl
         a32[3 * n3 + c] = (float)e;
         a32[7 * n3 + c] = (float)a[3 * n3 + c];
     }
+}
+
+// Finish kernels over a table: finish block b serves system map[b].x as its
+// block map[b].y of S.nfin.
+template <typename T, int F, int NV>
+__global__ void __launch_bounds__(kFinishBlock) k_finish_sys(const Sys<T> *__restrict__ sys,
+                                                             const int2 *__restrict__ map) {
+    __shared__ double lds[(kFinishBlock / 64) * 4];
+    const int2 e = map[blockIdx.x];
+    if (e.x < 0) return;
+    const Sys<T> &S = sys[e.x];
+    SolverState *st = S.st;
+    if (fin_skips<F>(st)) return;
+    double tot[NV];
+    if (finish_sum<NV>(S.partials, S.P.nblk, e.y, S.nfin, st->l2, &st->fin_counter, lds, tot) && threadIdx.x == 0)
+        fin_apply<F, NV>(st, tot, nullptr);
+}
+
+// Red-black line phase along J for every LINE system of a table (RSRS and
+// the 2-D R sweep): block b solves lines [map[b].y*64, +64) of colour
+// `param` of system map[b].x, numbered densely by color_line.  which = 0
+// preconditions P into Phat, 1 S into Shat (skipped after the S-norm exit).
+template <typename T>
+__global__ void __launch_bounds__(64) k_line_color_sys(const Sys<T> *__restrict__ sys, const int2 *__restrict__ map,
+                                                       int param, int which) {
+    const int2 e = map[blockIdx.x];
+    if (e.x < 0) return;
+    const Sys<T> &S = sys[e.x];
+    const SolverState *st = S.st;
+    if (st->done || (which == 1 && st->s_exit)) return;
+    const DGrid &g = S.g;
+    const LineGeom L = line_geom(g, AX_J);
+    int u, v;
+    if (!color_line(L.nu, L.nv, g.do_k ? 1 : 0, param, (int64_t)e.y * 64 + threadIdx.x, u, v)) return;
+    const int64_t c0 = (int64_t)u * L.su + (int64_t)v * L.sv;
+    T *sc = S.line_scratch;
+    const int64_t n3 = g.n3;
+    solve_line<T>(g, L, AX_J, c0, S.a, which == 0 ? S.p : S.s, which == 0 ? S.phat : S.shat, sc, sc + n3,
+                  sc + 2 * n3, sc + 3 * n3, 0, nullptr);
+}
+
+// Number of the listed systems still running.
+template <typename T>
+__global__ void k_count_running(const Sys<T> *__restrict__ sys, const int *__restrict__ ids, int nids, int *out) {
+    __shared__ int cnt;
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    for (int q = threadIdx.x; q < nids; q += blockDim.x)
+        if (!sys[ids[q]].st->done) atomicAdd(&cnt, 1);
+    __syncthreads();
+    if (threadIdx.x == 0) *out = cnt;
 }
 
 // ---- LEQ_MATVEC / LEQ_MSOLVE1 / dot / copy as standalone kernels ----------
@@ -312,16 +423,40 @@ __global__ void __launch_bounds__(BS) k_dot(DGrid g, Part P, const T *__restrict
     block_partial<BS, 1>(v, partials, P.nblk, blk, lds);
 }
 
+// Copy (LINE start value z = r of the standalone LEQ_MSOLVE entry,
+// leqsol_mod.f:341-357, and the traffic-calibration copy); with `st` it is
+// skipped once the solve is done or, with skip_on_sexit, after the S-norm
+// exit.
 template <int BS, typename T>
 __global__ void __launch_bounds__(BS) k_copy_if(Part P, const T *__restrict__ src, T *__restrict__ dst,
                                                 const SolverState *st, int skip_on_sexit) {
-    if (st) {
-        if (st->done) return;
-        if (skip_on_sexit && st->s_exit) return;
-    }
+    if (st && (st->done || (skip_on_sexit && st->s_exit))) return;
     if (!xcd_valid(P.nblk)) return;
     const int64_t blk = xcd_block(P.nblk);
     FOR_CELLS(BS, P, blk, c) dst[c] = src[c];
+}
+
+// One-system finish (standalone dot product).
+template <int F, int NV>
+__global__ void __launch_bounds__(kFinishBlock) k_finish(SolverState *st, const double *__restrict__ partials,
+                                                         int64_t nblk, double *out) {
+    __shared__ double lds[(kFinishBlock / 64) * 4];
+    if (fin_skips<F>(st)) return;
+    double tot[NV];
+    if (finish_sum<NV>(partials, nblk, (int)blockIdx.x, (int)gridDim.x, st->l2, &st->fin_counter, lds, tot) &&
+        threadIdx.x == 0)
+        fin_apply<F, NV>(st, tot, out);
+}
+
+// Blank state for the standalone dot/scale entries.
+static __global__ void k_init_state(SolverState *st) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        SolverState s = {};
+        s.icheck = 1;
+        s.itmax = 1;
+        s.true_rnorm = -1.0;
+        *st = s;
+    }
 }
 
 }  // namespace mfix

@@ -1,7 +1,11 @@
 // Solver engine of the gfx950 LEQ_BICGS path: device context, grow-only
-// workspaces, launch shapes (picked per grid size by the autotuner), the
-// FP64 driver that follows LEQ_BICGS0 (model/leq_bicgs.f:107-509) and the
-// mixed-precision driver (FP32 recurrence, FP64 reliable updates).
+// workspaces, launch shapes (picked per grid size by the autotuner), system
+// tables and their block maps, the launch sequence of LEQ_BICGS0
+// (model/leq_bicgs.f:107-509) over a table, the driver that polls the
+// running-system count between chunks of iterations (replaying a captured
+// HIP graph per chunk), and the single and batched solve entries built on
+// it.  The mixed-precision inner solve (FP32 recurrence, FP64 reliable
+// updates) runs the same sequence on a float table.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -13,7 +17,6 @@
 
 #include "../../include/mfix_leq.h"
 #include "leq_context.h"
-#include "leq_phases.h"
 
 namespace mfix {
 
@@ -66,7 +69,8 @@ int DevBuf::ensure(size_t bytes) {
     hipError_t e = hipMalloc(&ptr, bytes);
     if (e != hipSuccess) {
         ptr = nullptr;
-        return hip_fail(e, "hipMalloc");
+        hip_fail(e, "hipMalloc");
+        return MFIX_LEQ_ENOMEM;
     }
     cap = bytes;
     return 0;
@@ -77,6 +81,15 @@ void DevBuf::release() {
     ptr = nullptr;
     cap = 0;
 }
+
+template <typename T>
+void Group<T>::drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    exec = nullptr;
+    exec_iters = 0;
+}
+template struct Group<double>;
+template struct Group<float>;
 
 int Context::ensure_init() {
     if (initialised) return 0;
@@ -92,13 +105,23 @@ int Context::init(int dev) {
     if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
     e = hipHostMalloc((void **)&host_state, sizeof(SolverState) * kHostStates, hipHostMallocDefault);
     if (e != hipSuccess) return hip_fail(e, "hipHostMalloc");
+    e = hipHostMalloc((void **)&host_count, sizeof(int) * 4, hipHostMallocDefault);
+    if (e != hipSuccess) return hip_fail(e, "hipHostMalloc");
+    if (const char *v = std::getenv("MFIX_LEQ_GRAPHS")) use_graphs = std::atoi(v);
     initialised = true;
     return 0;
 }
 
 void Context::release_all() {
-    for (DevBuf *b : {&d_a, &d_b, &d_x, &ws64, &ws32, &line_scratch, &partials, &state, &a32, &xalt, &misc})
+    for (DevBuf *b : {&d_a, &d_b, &d_x, &ws, &bws, &partials, &state, &misc, &tmp}) b->release();
+    g64.drop_graph();
+    g32.drop_graph();
+    b64.drop_graph();
+    b32.drop_graph();
+    for (DevBuf *b : {&g64.dev, &g32.dev, &b64.dev, &b32.dev, &g64.iter.dev, &g32.iter.dev, &b64.iter.dev,
+                      &b32.iter.dev})
         b->release();
+    g64.iter.version = g32.iter.version = b64.iter.version = b32.iter.version = 0;
     for (auto &p : prof) {
         for (auto &ev : p.events) {
             (void)hipEventDestroy(ev.first);
@@ -174,26 +197,28 @@ int size_class(int64_t n3) {
 
 static inline dim3 grid_of(const Part &P) { return dim3((unsigned)xcd_grid(P.nblk)); }
 
-template <int F, int NV>
-void launch_finish(SolverState *st, const double *partials, int64_t nblk, hipStream_t s, double *out) {
+static inline int finish_blocks(int64_t nblk) {
     // about one partial per finish thread, at most kFinishBlocks blocks:
     // small grids finish in a single block without atomics
     const int64_t want = (nblk + kFinishBlock - 1) / kFinishBlock;
-    const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>(kFinishBlocks, want));
-    hipLaunchKernelGGL((k_finish<F, NV>), dim3(blocks), dim3(kFinishBlock), 0, s, st, partials, nblk, out);
+    return (int)std::max<int64_t>(1, std::min<int64_t>(kFinishBlocks, want));
 }
 
-// ---- LINE preconditioner (LEQ_MSOLVE, leqsol_mod.f:286-598) ---------------
+// ---- LINE preconditioner, one system (LEQ_MSOLVE, leqsol_mod.f:286-598) ----
 
 template <typename T>
 int launch_msolve_line(const DGrid &g, const Shape &sh, const char *cmethod, const T *a, const T *r, T *z,
-                       T *scratch, const SolverState *st, int skip_on_sexit, int *fatal_flag, hipStream_t s) {
+                       T *scratch, const SolverState *st, int skip_on_sexit, int *fatal_flag, hipStream_t s,
+                       bool copy_first) {
     const int64_t n3 = g.n3;
     T *sdl = scratch, *sd = scratch + n3, *sdu = scratch + 2 * n3, *sb = scratch + 3 * n3;
     Prof &pf = ctx().prof[1];
-    const Part P = make_part(n3, sh.bs, sh.cpt);
-    MFIX_DISPATCH_BS(sh.bs, BS,
-                     hipLaunchKernelGGL((k_copy_if<BS, T>), grid_of(P), dim3(BS), 0, s, P, r, z, st, skip_on_sexit));
+    if (copy_first) {
+        const Part P = make_part(n3, sh.bs, sh.cpt);
+        MFIX_DISPATCH_BS(sh.bs, BS,
+                         hipLaunchKernelGGL((k_copy_if<BS, T>), grid_of(P), dim3(BS), 0, s, P, r, z, st,
+                                            skip_on_sexit));
+    }
     const int lpb = sh.lines_per_block > 0 ? sh.lines_per_block : 64;
     auto run = [&](const LinePhase &ph, int64_t nthreads) {
         const unsigned blocks = (unsigned)((nthreads + lpb - 1) / lpb);
@@ -262,9 +287,9 @@ int launch_msolve_line(const DGrid &g, const Shape &sh, const char *cmethod, con
 }
 
 template int launch_msolve_line<double>(const DGrid &, const Shape &, const char *, const double *, const double *,
-                                        double *, double *, const SolverState *, int, int *, hipStream_t);
+                                        double *, double *, const SolverState *, int, int *, hipStream_t, bool);
 template int launch_msolve_line<float>(const DGrid &, const Shape &, const char *, const float *, const float *,
-                                       float *, float *, const SolverState *, int, int *, hipStream_t);
+                                       float *, float *, const SolverState *, int, int *, hipStream_t, bool);
 
 // Per-iteration algorithmic bytes per cell of the fused three-pass schedule.
 double alg_bytes_per_cell(int pc, int sa, int sv, int sx) {
@@ -273,85 +298,391 @@ double alg_bytes_per_cell(int pc, int sa, int sv, int sx) {
     return 42.0 * sa + 30.0 * sv + 2.0 * sx;
 }
 
-// One BiCGSTAB iteration (leq_bicgs.f:316-466) of the plain FP64 or the inner
-// FP32 solve.
-template <int BS, typename T>
-static void iteration(const SolveBufs<T> &w, const Shape &sh, int pc, const char *cmethod, hipStream_t s) {
-    const DGrid &g = w.g;
-    const Part P = make_part(g.n3, BS, sh.cpt);
-    const dim3 grid = grid_of(P), blk(BS);
-    Prof &pf = ctx().prof[0];
-    const T *a0 = w.a + (int64_t)DiagPlane<T>::value * g.n3;
-    // algorithmic bytes of one stencil launch: 7 coefficient planes, the
-    // operand, the dot partner and the result, once each
-    const double mv_bytes = (double)g.n3 * (10.0 * sizeof(T));
-    hipLaunchKernelGGL((k_p_update<BS, T>), grid, blk, 0, s, g, P, w.r, w.p, w.v, w.rt, w.st);
-    const T *phat = w.p;
-    if (pc == PC_LINE) {
-        launch_msolve_line<T>(g, sh, cmethod, w.a, w.p, w.phat, w.line_scratch, w.st, 0, &w.st->msolve_fatal, s);
-        phat = w.phat;
-    }
-    pf.begin(s);
-    if (pc == PC_DIAG)
-        hipLaunchKernelGGL((k_matvec_dots<BS, T, 0, true>), grid, blk, 0, s, g, P, w.a, phat, w.v, w.rt, w.st,
-                           w.partials);
-    else
-        hipLaunchKernelGGL((k_matvec_dots<BS, T, 0, false>), grid, blk, 0, s, g, P, w.a, phat, w.v, w.rt, w.st,
-                           w.partials);
-    pf.end(s, mv_bytes);
-    launch_finish<FIN_ALPHA, 1>(w.st, w.partials, P.nblk, s, nullptr);
-    hipLaunchKernelGGL((k_s_update<BS, T>), grid, blk, 0, s, g, P, w.r, w.v, w.s, w.st, w.partials);
-    launch_finish<FIN_SNORM, 1>(w.st, w.partials, P.nblk, s, nullptr);
-    const T *shat = w.s;
-    if (pc == PC_LINE) {
-        launch_msolve_line<T>(g, sh, cmethod, w.a, w.s, w.shat, w.line_scratch, w.st, 1, &w.st->msolve_fatal, s);
-        shat = w.shat;
-    }
-    pf.begin(s);
-    if (pc == PC_DIAG)
-        hipLaunchKernelGGL((k_matvec_dots<BS, T, 1, true>), grid, blk, 0, s, g, P, w.a, shat, w.t, w.s, w.st,
-                           w.partials);
-    else
-        hipLaunchKernelGGL((k_matvec_dots<BS, T, 1, false>), grid, blk, 0, s, g, P, w.a, shat, w.t, w.s, w.st,
-                           w.partials);
-    pf.end(s, mv_bytes);
-    launch_finish<FIN_OMEGA, 2>(w.st, w.partials, P.nblk, s, nullptr);
-    if (pc == PC_DIAG)
-        hipLaunchKernelGGL((k_xr_update<BS, T, true>), grid, blk, 0, s, g, P, w.x, (const T *)w.p, (const T *)w.s, a0,
-                           w.s, w.t, w.r, w.rt, w.st, w.partials);
-    else
-        hipLaunchKernelGGL((k_xr_update<BS, T, false>), grid, blk, 0, s, g, P, w.x, phat, shat, a0, w.s, w.t, w.r,
-                           w.rt, w.st, w.partials);
-    if (w.mixed) launch_finish<FIN_XR_MIXED, 2>(w.st, w.partials, P.nblk, s, nullptr);
-    else launch_finish<FIN_XR, 2>(w.st, w.partials, P.nblk, s, nullptr);
+// ---- system workspaces --------------------------------------------------------
+
+static inline size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+// FP64: R, Rtilde, P, S, T, V (+ Phat, Shat and 4 DGTSV scratch planes for
+// LINE).  Mixed: the FP32 r, rt, p, s, t, v, y (+ LINE arrays), the 8-plane
+// FP32 matrix, the FP64 residual and second solution buffer.  Both: the
+// partial sums and the solver state.
+size_t sys_bytes(const DGrid &g, const Part &P, int pc, bool mixed) {
+    const size_t n3 = (size_t)g.n3;
+    const size_t sv = mixed ? sizeof(float) : sizeof(double);
+    const bool line = pc == PC_LINE;
+    size_t b = 0;
+    b += (size_t)(mixed ? 7 : 6) * align256(n3 * sv);
+    if (line) b += 2 * align256(n3 * sv) + align256(4 * n3 * sv);
+    if (mixed) b += align256(8 * n3 * sizeof(float)) + 2 * align256(n3 * sizeof(double));
+    b += align256(sizeof(double) * kMaxPartials * (size_t)P.nblk);
+    b += align256(sizeof(SolverState));
+    return b;
 }
 
 template <typename T>
-void enqueue_iteration(const SolveBufs<T> &w, const Shape &sh, int pc, const char *cmethod, hipStream_t s) {
-    MFIX_DISPATCH_BS(sh.bs, BS, iteration<BS, T>(w, sh, pc, cmethod, s));
+void sys_carve(Sys<T> &S, const SysDesc &d, const Part &P, char *base) {
+    const bool mixed = sizeof(T) == sizeof(float);
+    const size_t n3 = (size_t)d.g.n3;
+    char *q = base;
+    auto take = [&](size_t bytes) {
+        char *p = q;
+        q += align256(bytes);
+        return p;
+    };
+    S = Sys<T>{};
+    S.g = d.g;
+    S.P = P;
+    S.nfin = finish_blocks(P.nblk);
+    S.pc = d.pc;
+    S.itmax = d.itmax;
+    S.minimize = d.minimize;
+    S.icheck = d.icheck;
+    S.mixed = mixed ? 1 : 0;
+    S.tol = d.tol;
+    S.delta = d.delta;
+    S.r = (T *)take(n3 * sizeof(T));
+    S.rt = (T *)take(n3 * sizeof(T));
+    S.p = (T *)take(n3 * sizeof(T));
+    S.s = (T *)take(n3 * sizeof(T));
+    S.t = (T *)take(n3 * sizeof(T));
+    S.v = (T *)take(n3 * sizeof(T));
+    if (mixed) S.x = (T *)take(n3 * sizeof(T));
+    else S.x = (T *)d.x;
+    if (d.pc == PC_LINE) {
+        S.phat = (T *)take(n3 * sizeof(T));
+        S.shat = (T *)take(n3 * sizeof(T));
+        S.line_scratch = (T *)take(4 * n3 * sizeof(T));
+    }
+    if (mixed) {
+        S.a = (const T *)take(8 * n3 * sizeof(float));
+        S.r64 = (double *)take(n3 * sizeof(double));
+        S.xb = (double *)take(n3 * sizeof(double));
+    } else {
+        S.a = (const T *)d.a;
+        S.r64 = (double *)S.r;
+        S.xb = nullptr;
+    }
+    S.partials = (double *)take(sizeof(double) * kMaxPartials * (size_t)P.nblk);
+    S.st = (SolverState *)take(sizeof(SolverState));
+    S.a64 = d.a;
+    S.b64 = d.b;
+    S.xa = d.x;
+    S.noise = d.noise;
+    S.seed = d.seed;
 }
-template void enqueue_iteration<double>(const SolveBufs<double> &, const Shape &, int, const char *, hipStream_t);
-template void enqueue_iteration<float>(const SolveBufs<float> &, const Shape &, int, const char *, hipStream_t);
+template void sys_carve<double>(Sys<double> &, const SysDesc &, const Part &, char *);
+template void sys_carve<float>(Sys<float> &, const SysDesc &, const Part &, char *);
 
-// Poll the device state every few iterations until the solve is done.  The
-// chunk doubles up to 64; launches after convergence return immediately.
-int drive(const std::function<void()> &one_iteration, SolverState *d_state, SolverState *h_state, int itmax,
-          hipStream_t s) {
-    int issued = 0;
-    int chunk = 4;
-    for (;;) {
-        hipError_t e = hipMemcpyAsync(h_state, d_state, sizeof(SolverState), hipMemcpyDeviceToHost, s);
-        if (e != hipSuccess) return hip_fail(e, "state readback");
-        e = hipStreamSynchronize(s);
-        if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
-        if (h_state->done) return 0;
-        if (issued > 4 * (itmax + 8)) return hip_fail(hipErrorUnknown, "solver did not terminate");
-        const int todo = std::min(chunk, std::max(1, itmax + 2 - h_state->i));
-        for (int q = 0; q < todo; ++q) one_iteration();
-        issued += todo;
-        chunk = std::min(chunk * 2, 64);
+// ---- block maps and table upload ----------------------------------------------
+
+// Which LINE sweeps of a system run as batched red-black passes: every
+// character is R (a pass) or a character that does nothing in this
+// dimension (S, and J; K and A too in 2-D).  Returns the number of passes,
+// or -1 when the system needs the one-by-one sweep (I/K wavefronts, A).
+static int rb_passes(const Sweep &sw, bool do_k) {
+    int n = 0;
+    for (char ch : sw) {
+        if (ch == 'R' || ch == 'r') ++n;
+        else if (ch == 'I' || ch == 'i') return -1;
+        else if (do_k && (ch == 'K' || ch == 'k' || ch == 'A' || ch == 'a')) return -1;
+    }
+    return n;
+}
+
+template <typename T>
+int build_maps(const Group<T> &G, const std::vector<int> &ids, MapSet &M, hipStream_t s) {
+    std::vector<int2> smap, fmap, lmap[kMaxRPasses];
+    std::vector<int> seq;
+    int nlpass = 0;
+    double cells = 0.0;
+    for (int id : ids) {
+        const Sys<T> &S = G.sys[id];
+        cells += (double)S.g.n3;
+        // stream kernels: an 8-aligned range whose slot j holds the logical
+        // block that puts one contiguous slab of the system on each XCD
+        const int64_t nb = S.P.nblk;
+        const int64_t per = (nb + kNumXcd - 1) / kNumXcd;
+        const int64_t full = nb - (per - 1) * kNumXcd;
+        for (int64_t j = 0; j < per * kNumXcd; ++j) {
+            const int64_t x = j % kNumXcd, r = j / kNumXcd;
+            const bool valid = r < (x < full ? per : per - 1);
+            const int64_t start = x < full ? x * per : full * per + (x - full) * (per - 1);
+            smap.push_back(valid ? make_int2(id, (int)(start + r)) : make_int2(-1, -1));
+        }
+        for (int f = 0; f < S.nfin; ++f) fmap.push_back(make_int2(id, f));
+        if (S.pc == PC_LINE) {
+            const int np = rb_passes(G.sweep[id], S.g.do_k != 0);
+            if (np < 0) {
+                seq.push_back(id);
+            } else {
+                const LineGeom L = line_geom(S.g, AX_J);
+                const int64_t lines = ((int64_t)L.nu * L.nv + 1) / 2;
+                const int64_t lb = (lines + 63) / 64;
+                for (int q = 0; q < np; ++q)
+                    for (int64_t b = 0; b < lb; ++b) lmap[q].push_back(make_int2(id, (int)b));
+                nlpass = std::max(nlpass, np);
+            }
+        }
+    }
+    bool same = M.ids == ids && M.smap.size() == smap.size() && M.fmap.size() == fmap.size() &&
+                M.nlpass == nlpass && M.line_seq == seq && M.dev.ptr;
+    if (same) same = !std::memcmp(M.smap.data(), smap.data(), smap.size() * sizeof(int2)) &&
+                     !std::memcmp(M.fmap.data(), fmap.data(), fmap.size() * sizeof(int2));
+    for (int q = 0; q < kMaxRPasses && same; ++q)
+        same = M.lmap[q].size() == lmap[q].size() &&
+               !std::memcmp(M.lmap[q].data(), lmap[q].data(), lmap[q].size() * sizeof(int2));
+    M.cells = cells;
+    if (same) return 0;
+    M.ids = ids;
+    M.smap.swap(smap);
+    M.fmap.swap(fmap);
+    for (int q = 0; q < kMaxRPasses; ++q) M.lmap[q].swap(lmap[q]);
+    M.nlpass = nlpass;
+    M.line_seq.swap(seq);
+    // one device block: ids | smap | fmap | lmap[0..3]
+    std::vector<char> blob;
+    auto put = [&](const void *p, size_t bytes) {
+        const size_t off = blob.size();
+        blob.resize(align256(off + bytes));
+        if (bytes) std::memcpy(blob.data() + off, p, bytes);
+        return off;
+    };
+    const size_t o_ids = put(M.ids.data(), M.ids.size() * sizeof(int));
+    const size_t o_s = put(M.smap.data(), M.smap.size() * sizeof(int2));
+    const size_t o_f = put(M.fmap.data(), M.fmap.size() * sizeof(int2));
+    size_t o_l[kMaxRPasses];
+    for (int q = 0; q < kMaxRPasses; ++q) o_l[q] = put(M.lmap[q].data(), M.lmap[q].size() * sizeof(int2));
+    int rc;
+    if ((rc = M.dev.ensure(std::max<size_t>(blob.size(), 256)))) return rc;
+    hipError_t e = hipMemcpyAsync(M.dev.ptr, blob.data(), blob.size(), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(e, "block map upload");
+    char *base = (char *)M.dev.ptr;
+    M.d_ids = (int *)(base + o_ids);
+    M.d_smap = (int2 *)(base + o_s);
+    M.d_fmap = (int2 *)(base + o_f);
+    for (int q = 0; q < kMaxRPasses; ++q) M.d_lmap[q] = (int2 *)(base + o_l[q]);
+    M.version += 1;
+    return 0;
+}
+template int build_maps<double>(const Group<double> &, const std::vector<int> &, MapSet &, hipStream_t);
+template int build_maps<float>(const Group<float> &, const std::vector<int> &, MapSet &, hipStream_t);
+
+template <typename T>
+int upload_table(Group<T> &G, hipStream_t s) {
+    int rc;
+    const size_t bytes = G.sys.size() * sizeof(Sys<T>);
+    if ((rc = G.dev.ensure(std::max<size_t>(bytes, 256)))) return rc;
+    if (G.d_sys != (Sys<T> *)G.dev.ptr) G.drop_graph();
+    G.d_sys = (Sys<T> *)G.dev.ptr;
+    if (!bytes) return 0;
+    hipError_t e = hipMemcpyAsync(G.d_sys, G.sys.data(), bytes, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    return e == hipSuccess ? 0 : hip_fail(e, "system table upload");
+}
+template int upload_table<double>(Group<double> &, hipStream_t);
+template int upload_table<float>(Group<float> &, hipStream_t);
+
+// ---- launch sequences -------------------------------------------------------------
+
+template <typename T, int F, int NV>
+static void finish(const Group<T> &G, const MapSet &M, hipStream_t s) {
+    hipLaunchKernelGGL((k_finish_sys<T, F, NV>), dim3((unsigned)M.fmap.size()), dim3(kFinishBlock), 0, s, G.d_sys,
+                       M.d_fmap);
+}
+
+template <int BS, typename T>
+static void setup_bs(Group<T> &G, const MapSet &M, hipStream_t s) {
+    const dim3 gs((unsigned)M.smap.size()), blk(BS);
+    const int nids = (int)M.ids.size();
+    hipLaunchKernelGGL(k_init_states<T>, dim3((unsigned)((nids + 255) / 256)), dim3(256), 0, s, G.d_sys, M.d_ids,
+                       nids);
+    hipLaunchKernelGGL((k_setup_residual<BS, T>), gs, blk, 0, s, G.d_sys, M.d_smap);
+    finish<T, FIN_SETUP, 1>(G, M, s);
+    if (G.mixed) hipLaunchKernelGGL((k_prepare_a32<BS>), gs, blk, 0, s, (const Sys<float> *)G.d_sys, M.d_smap);
+    hipLaunchKernelGGL((k_shadow<BS, T>), gs, blk, 0, s, G.d_sys, M.d_smap);
+    finish<T, FIN_SHADOW, 1>(G, M, s);
+}
+
+template <typename T>
+void enqueue_setup(Group<T> &G, const MapSet &M, hipStream_t s) {
+    if (M.ids.empty()) return;
+    MFIX_DISPATCH_BS(G.bs, BS, setup_bs<BS, T>(G, M, s));
+}
+template void enqueue_setup<double>(Group<double> &, const MapSet &, hipStream_t);
+template void enqueue_setup<float>(Group<float> &, const MapSet &, hipStream_t);
+
+// LINE preconditioner of every LINE system of the table: batched red-black
+// passes, then the systems swept one by one.  The start value (Phat = P,
+// Shat = S) was written by the producing kernel.
+template <typename T>
+static void line_sweeps(const Group<T> &G, const MapSet &M, int which, hipStream_t s) {
+    Prof &pf = ctx().prof[1];
+    for (int q = 0; q < M.nlpass; ++q) {
+        if (M.lmap[q].empty()) continue;
+        for (int param : {1, 0}) {
+            pf.begin(s);
+            hipLaunchKernelGGL(k_line_color_sys<T>, dim3((unsigned)M.lmap[q].size()), dim3(64), 0, s, G.d_sys,
+                               M.d_lmap[q], param, which);
+            pf.end(s, 0.5 * M.cells * 13.0 * sizeof(T));
+        }
+    }
+    for (int id : M.line_seq) {
+        const Sys<T> &S = G.sys[id];
+        const Shape sh{G.bs, 1, 64};
+        launch_msolve_line<T>(S.g, sh, G.sweep[id].data(), S.a, which == 0 ? S.p : S.s,
+                              which == 0 ? S.phat : S.shat, S.line_scratch, S.st, which, &S.st->msolve_fatal, s,
+                              false);
     }
 }
+
+// One BiCGSTAB iteration (leq_bicgs.f:316-466) over the table.
+template <int BS, typename T>
+static void iteration_bs(const Group<T> &G, const MapSet &M, hipStream_t s) {
+    const dim3 gs((unsigned)M.smap.size()), blk(BS);
+    Prof &pf = ctx().prof[0];
+    // algorithmic bytes of one stencil launch: 7 coefficient planes, the
+    // operand, the dot partner and the result, once each
+    const double mv_bytes = M.cells * (10.0 * sizeof(T));
+    const bool any_line = M.nlpass > 0 || !M.line_seq.empty();
+    hipLaunchKernelGGL((k_p_update<BS, T>), gs, blk, 0, s, G.d_sys, M.d_smap);
+    if (any_line) line_sweeps(G, M, 0, s);
+    pf.begin(s);
+    hipLaunchKernelGGL((k_matvec_dots<BS, T, 0>), gs, blk, 0, s, G.d_sys, M.d_smap);
+    pf.end(s, mv_bytes);
+    finish<T, FIN_ALPHA, 1>(G, M, s);
+    hipLaunchKernelGGL((k_s_update<BS, T>), gs, blk, 0, s, G.d_sys, M.d_smap);
+    finish<T, FIN_SNORM, 1>(G, M, s);
+    if (any_line) line_sweeps(G, M, 1, s);
+    pf.begin(s);
+    hipLaunchKernelGGL((k_matvec_dots<BS, T, 1>), gs, blk, 0, s, G.d_sys, M.d_smap);
+    pf.end(s, mv_bytes);
+    finish<T, FIN_OMEGA, 2>(G, M, s);
+    hipLaunchKernelGGL((k_xr_update<BS, T>), gs, blk, 0, s, G.d_sys, M.d_smap);
+    if (G.mixed) {
+        finish<T, FIN_XR_MIXED, 2>(G, M, s);
+        hipLaunchKernelGGL((k_reliable_update<BS>), gs, blk, 0, s, (const Sys<float> *)G.d_sys, M.d_smap);
+        finish<T, FIN_UPDATE, 3>(G, M, s);
+    } else {
+        finish<T, FIN_XR, 2>(G, M, s);
+    }
+}
+
+template <typename T>
+void enqueue_iterations(Group<T> &G, const MapSet &M, int n, hipStream_t s) {
+    if (M.ids.empty()) return;
+    MFIX_DISPATCH_BS(G.bs, BS, for (int q = 0; q < n; ++q) iteration_bs<BS, T>(G, M, s));
+}
+template void enqueue_iterations<double>(Group<double> &, const MapSet &, int, hipStream_t);
+template void enqueue_iterations<float>(Group<float> &, const MapSet &, int, hipStream_t);
+
+template <typename T>
+static void enqueue_count(const Group<T> &G, const MapSet &M, int *d_out, hipStream_t s) {
+    hipLaunchKernelGGL(k_count_running<T>, dim3(1), dim3(256), 0, s, G.d_sys, M.d_ids, (int)M.ids.size(), d_out);
+}
+
+// Chunk of iterations of both tables, captured once and replayed.
+struct ChunkGraph {
+    hipGraphExec_t exec = nullptr;
+    int iters = 0;
+    uint64_t v64 = 0, v32 = 0;
+    const void *s64 = nullptr, *s32 = nullptr;
+    const void *m64 = nullptr, *m32 = nullptr;
+};
+
+static int capture_chunk(Group<double> *G64, Group<float> *G32, int iters, hipStream_t s, hipGraphExec_t *out) {
+    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
+    for (int q = 0; q < iters; ++q) {
+        if (G64) enqueue_iterations<double>(*G64, G64->iter, 1, s);
+        if (G32) enqueue_iterations<float>(*G32, G32->iter, 1, s);
+    }
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(s, &graph);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    e = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+    return 0;
+}
+
+// Drive both tables (either may be null) until every system is done.  The
+// running count is polled after each chunk; chunks grow 4, 8, ... 64
+// iterations (direct launches) or are a fixed captured graph.  Launches
+// after a system is done return at once, so overshooting is harmless.
+static int run_tables(Group<double> *G64, Group<float> *G32, int max_iters, hipStream_t s, bool graphs) {
+    Context &C = ctx();
+    int rc;
+    if ((rc = C.misc.ensure(4096))) return rc;
+    int *d_cnt = (int *)C.misc.ptr;
+    if (G64 && G64->iter.ids.empty()) G64 = nullptr;
+    if (G32 && G32->iter.ids.empty()) G32 = nullptr;
+    if (!G64 && !G32) return 0;
+    if (C.prof[0].on || C.prof[1].on) graphs = false;
+    // one-by-one LINE sweeps bake their arrays into the launches: no replay
+    if ((G64 && !G64->iter.line_seq.empty()) || (G32 && !G32->iter.line_seq.empty())) graphs = false;
+    // graph owner: the float table's slot when present, else the double one
+    hipGraphExec_t *exec = G32 ? &G32->exec : &G64->exec;
+    int *exec_iters = G32 ? &G32->exec_iters : &G64->exec_iters;
+    uint64_t *exec_version = G32 ? &G32->exec_version : &G64->exec_version;
+    const void **exec_sys = G32 ? &G32->exec_sys : &G64->exec_sys;
+    const uint64_t version = (G64 ? G64->iter.version * 1000003ull : 0) ^ (G32 ? G32->iter.version : 0) ^
+                             ((G64 ? 1ull : 0) << 62) ^ ((G32 ? 1ull : 0) << 63);
+    const void *sys_key = G32 ? (const void *)G32->d_sys : (const void *)G64->d_sys;
+    const int kGraphIters = 16;
+    int64_t issued = 0;
+    int chunk = 4;
+    for (;;) {
+        if (G64) enqueue_count<double>(*G64, G64->iter, d_cnt + 0, s);
+        if (G32) enqueue_count<float>(*G32, G32->iter, d_cnt + 1, s);
+        hipError_t e = hipMemcpyAsync(C.host_count, d_cnt, 2 * sizeof(int), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return hip_fail(e, "running count");
+        const int running = (G64 ? C.host_count[0] : 0) + (G32 ? C.host_count[1] : 0);
+        if (running == 0) return 0;
+        if (issued > 4 * ((int64_t)max_iters + 8)) return hip_fail(hipErrorUnknown, "solver did not terminate");
+        if (graphs) {
+            if (!*exec || *exec_iters != kGraphIters || *exec_version != version || *exec_sys != sys_key) {
+                if (*exec) (void)hipGraphExecDestroy(*exec);
+                *exec = nullptr;
+                if ((rc = capture_chunk(G64, G32, kGraphIters, s, exec))) return rc;
+                *exec_iters = kGraphIters;
+                *exec_version = version;
+                *exec_sys = sys_key;
+            }
+            e = hipGraphLaunch(*exec, s);
+            if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
+            issued += kGraphIters;
+        } else {
+            for (int q = 0; q < chunk; ++q) {
+                if (G64) enqueue_iterations<double>(*G64, G64->iter, 1, s);
+                if (G32) enqueue_iterations<float>(*G32, G32->iter, 1, s);
+            }
+            issued += chunk;
+            chunk = std::min(chunk * 2, 64);
+        }
+    }
+}
+
+template <typename T>
+int run_group(Group<T> &G, int max_iters, hipStream_t s, bool graphs) {
+    if constexpr (sizeof(T) == sizeof(double)) return run_tables(&G, nullptr, max_iters, s, graphs);
+    else return run_tables(nullptr, &G, max_iters, s, graphs);
+}
+template int run_group<double>(Group<double> &, int, hipStream_t, bool);
+template int run_group<float>(Group<float> &, int, hipStream_t, bool);
+
+template <typename T>
+int read_states(const Group<T> &G, const std::vector<int> &ids, SolverState *h, hipStream_t s) {
+    for (size_t q = 0; q < ids.size(); ++q) {
+        hipError_t e = hipMemcpyAsync(h + q, G.sys[ids[q]].st, sizeof(SolverState), hipMemcpyDeviceToHost, s);
+        if (e != hipSuccess) return hip_fail(e, "state readback");
+    }
+    hipError_t e = hipStreamSynchronize(s);
+    return e == hipSuccess ? 0 : hip_fail(e, "state readback");
+}
+template int read_states<double>(const Group<double> &, const std::vector<int> &, SolverState *, hipStream_t);
+template int read_states<float>(const Group<float> &, const std::vector<int> &, SolverState *, hipStream_t);
 
 void fill_stats(const SolverState &h, mfix_leq_stats *st) {
     if (!st) return;
@@ -369,56 +700,36 @@ void fill_stats(const SolverState &h, mfix_leq_stats *st) {
     st->inner_iterations = h.mixed ? st->iterations : 0;
 }
 
-// ---- FP64 LEQ_BICGS on device arrays --------------------------------------
+// ---- single solves (a table of one) -------------------------------------------
 
-int solve_fp64(const DGrid &g, int pc, const char *cmethod, double tol, int itmax, double *x, double *a, double *b,
-               const mfix_leq_opts *opts, mfix_leq_stats *stats, hipStream_t s, int minimize, int icheck) {
+static Sweep make_sweep(const char *cmethod) {
+    Sweep sw = {'R', 'S', 'R', 'S'};
+    if (cmethod)
+        for (int q = 0; q < 4 && cmethod[q]; ++q) sw[q] = cmethod[q];
+    return sw;
+}
+
+template <typename T>
+static int solve_one(Group<T> &G, const SysDesc &d, const Shape &sh, mfix_leq_stats *stats, hipStream_t s) {
     Context &C = ctx();
-    const int64_t n3 = g.n3;
-    const Shape sh = C.shape_for(n3, false);
-    const Part P = make_part(n3, sh.bs, sh.cpt);
+    const Part P = make_part(d.g.n3, sh.bs, sh.cpt);
+    const bool mixed = sizeof(T) == sizeof(float);
     int rc;
-    // R, Rtilde, P, S, T, V, and for LINE Phat, Shat
-    const int nvec = pc == PC_LINE ? 8 : 6;
-    if ((rc = C.ws64.ensure(sizeof(double) * n3 * nvec))) return rc;
-    if ((rc = C.partials.ensure(sizeof(double) * P.nblk * kMaxPartials + 64))) return rc;
-    if ((rc = C.state.ensure(sizeof(SolverState)))) return rc;
-    if (pc == PC_LINE && (rc = C.line_scratch.ensure(sizeof(double) * n3 * 4))) return rc;
-    double *W = (double *)C.ws64.ptr;
-    SolveBufs<double> w;
-    w.g = g;
-    w.a = a;
-    w.x = x;
-    w.r = W + 0 * n3;
-    w.rt = W + 1 * n3;
-    w.p = W + 2 * n3;
-    w.s = W + 3 * n3;
-    w.t = W + 4 * n3;
-    w.v = W + 5 * n3;
-    w.phat = pc == PC_LINE ? W + 6 * n3 : nullptr;
-    w.shat = pc == PC_LINE ? W + 7 * n3 : nullptr;
-    w.st = (SolverState *)C.state.ptr;
-    w.partials = (double *)C.partials.ptr;
-    w.line_scratch = pc == PC_LINE ? (double *)C.line_scratch.ptr : nullptr;
+    if ((rc = C.ws.ensure(sys_bytes(d.g, P, d.pc, mixed)))) return rc;
+    G.sys.resize(1);
+    G.sweep.assign(1, d.sweep);
+    G.bs = sh.bs;
+    G.mixed = mixed;
+    sys_carve<T>(G.sys[0], d, P, (char *)C.ws.ptr);
+    if ((rc = upload_table<T>(G, s))) return rc;
+    if ((rc = build_maps<T>(G, std::vector<int>{0}, G.iter, s))) return rc;
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
     (void)hipEventRecord(e0, s);
-    // k_init_state rewrites the whole state, so nothing carries over
-    hipLaunchKernelGGL(k_init_state, dim3(1), dim3(1), 0, s, w.st, tol, itmax, minimize, icheck, 0, 0.0);
-    const double *noise = opts ? opts->noise : nullptr;
-    const uint64_t seed = opts ? opts->noise_seed : 0;
-    MFIX_DISPATCH_BS(sh.bs, BS, {
-        hipLaunchKernelGGL((k_setup_residual<BS>), grid_of(P), dim3(BS), 0, s, g, P, a, b, (const double *)x, w.r,
-                           w.st, w.partials);
-        launch_finish<FIN_SETUP, 1>(w.st, w.partials, P.nblk, s, nullptr);
-        hipLaunchKernelGGL((k_shadow<BS, double>), grid_of(P), dim3(BS), 0, s, g, P, (const double *)w.r, w.r, w.rt,
-                           (double *)nullptr, noise, seed, w.st, w.partials);
-        launch_finish<FIN_SHADOW, 1>(w.st, w.partials, P.nblk, s, nullptr);
-    });
-    auto one = [&]() { enqueue_iteration<double>(w, sh, pc, cmethod, s); };
-    SolverState *h = C.host_state;
-    rc = drive(one, w.st, h, itmax, s);
+    enqueue_setup<T>(G, G.iter, s);
+    const bool graphs = C.use_graphs && d.g.n3 <= ((int64_t)1 << 22);
+    rc = run_group<T>(G, d.itmax, s, graphs);
     (void)hipEventRecord(e1, s);
     (void)hipEventSynchronize(e1);
     float ms = 0.f;
@@ -427,6 +738,13 @@ int solve_fp64(const DGrid &g, int pc, const char *cmethod, double tol, int itma
     (void)hipEventDestroy(e1);
     for (auto &p : C.prof) p.collect();
     if (rc) return rc;
+    SolverState *h = C.host_state;
+    if ((rc = read_states<T>(G, G.iter.ids, h, s))) return rc;
+    if (mixed && h->x_cur == 1) {
+        hipError_t e = hipMemcpyAsync(d.x, G.sys[0].xb, sizeof(double) * d.g.n3, hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return hip_fail(e, "solution copy");
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "kernel launch");
     fill_stats(*h, stats);
@@ -435,75 +753,110 @@ int solve_fp64(const DGrid &g, int pc, const char *cmethod, double tol, int itma
     return 0;
 }
 
-// ---- mixed precision (FP32 recurrence, FP64 reliable updates) -------------
+static SysDesc make_desc(const DGrid &g, int pc, const char *cmethod, double tol, int itmax, double *x, double *a,
+                         double *b, const mfix_leq_opts *opts, int minimize, int icheck, double delta) {
+    SysDesc d;
+    d.g = g;
+    d.pc = pc;
+    d.sweep = make_sweep(cmethod);
+    d.tol = tol;
+    d.itmax = itmax;
+    d.minimize = minimize;
+    d.icheck = icheck;
+    d.delta = delta;
+    d.x = x;
+    d.a = a;
+    d.b = b;
+    d.noise = opts ? opts->noise : nullptr;
+    d.seed = opts ? opts->noise_seed : 0;
+    return d;
+}
+
+int solve_fp64(const DGrid &g, int pc, const char *cmethod, double tol, int itmax, double *x, double *a, double *b,
+               const mfix_leq_opts *opts, mfix_leq_stats *stats, hipStream_t s, int minimize, int icheck) {
+    const SysDesc d = make_desc(g, pc, cmethod, tol, itmax, x, a, b, opts, minimize, icheck, 0.0);
+    return solve_one<double>(ctx().g64, d, ctx().shape_for(g.n3, false), stats, s);
+}
 
 int solve_mixed(const DGrid &g, int pc, const char *cmethod, double tol, int itmax, double *x, double *a, double *b,
                 const mfix_leq_opts *opts, mfix_leq_stats *stats, hipStream_t s) {
-    Context &C = ctx();
-    const int64_t n3 = g.n3;
-    const Shape sh = C.shape_for(n3, true);
-    const Part P = make_part(n3, sh.bs, sh.cpt);
-    int rc;
-    // FP32: r, rt, p, s, t, v, y (+ phat, shat for LINE); FP64: r64, second x
-    const int nvec = pc == PC_LINE ? 9 : 7;
-    if ((rc = C.ws32.ensure(sizeof(float) * n3 * nvec))) return rc;
-    if ((rc = C.a32.ensure(sizeof(float) * n3 * 8))) return rc;
-    if ((rc = C.xalt.ensure(sizeof(double) * n3 * 2))) return rc;
-    if ((rc = C.partials.ensure(sizeof(double) * P.nblk * kMaxPartials + 64))) return rc;
-    if ((rc = C.state.ensure(sizeof(SolverState)))) return rc;
-    if (pc == PC_LINE && (rc = C.line_scratch.ensure(sizeof(float) * n3 * 4))) return rc;
-    float *W = (float *)C.ws32.ptr;
-    float *A32 = (float *)C.a32.ptr;
-    double *r64 = (double *)C.xalt.ptr;
-    double *xb = r64 + n3;  // second FP64 solution buffer
-    SolveBufs<float> w;
-    w.g = g;
-    w.a = A32;
-    w.r = W + 0 * n3;
-    w.rt = W + 1 * n3;
-    w.p = W + 2 * n3;
-    w.s = W + 3 * n3;
-    w.t = W + 4 * n3;
-    w.v = W + 5 * n3;
-    w.x = W + 6 * n3;  // y: FP32 correction accumulator
-    w.phat = pc == PC_LINE ? W + 7 * n3 : nullptr;
-    w.shat = pc == PC_LINE ? W + 8 * n3 : nullptr;
-    w.st = (SolverState *)C.state.ptr;
-    w.partials = (double *)C.partials.ptr;
-    w.line_scratch = pc == PC_LINE ? (float *)C.line_scratch.ptr : nullptr;
-    w.mixed = true;
     const double delta = opts && opts->inner_tol > 0.0 ? opts->inner_tol : 0.1;
+    const SysDesc d = make_desc(g, pc, cmethod, tol, itmax, x, a, b, opts, 0, 1, delta);
+    return solve_one<float>(ctx().g32, d, ctx().shape_for(g.n3, true), stats, s);
+}
+
+// ---- batched solves ---------------------------------------------------------------
+
+// Every item becomes a row of the FP64 or the FP32 table; both tables
+// advance together, one launch per phase covering all their systems, in
+// captured chunks.  Threads per block are shared by a table; cells per
+// thread follow each system's size.
+int solve_batch(int nsys, const mfix_leq_batch_item *items, mfix_leq_stats *stats, hipStream_t s) {
+    Context &C = ctx();
+    int rc;
+    std::vector<SysDesc> descs((size_t)nsys);
+    std::vector<Part> parts((size_t)nsys);
+    std::vector<size_t> offs((size_t)nsys);
+    size_t total = 0;
+    const int bs = 256;
+    for (int q = 0; q < nsys; ++q) {
+        const mfix_leq_batch_item &it = items[q];
+        mfix_leq_grid hg = {};
+        hg.imax = it.imax;
+        hg.jmax = it.jmax;
+        hg.kmax = it.kmax;
+        hg.is_serial = 1;
+        hg.icheck_bicgs = 1;
+        const DGrid g = make_dgrid(hg);
+        const int pc = parse_pc(it.pc);
+        char cm[5] = {it.cmethod[0], it.cmethod[1], it.cmethod[2], it.cmethod[3], 0};
+        descs[q] = make_desc(g, pc, cm, it.tol, it.itmax, it.d_var, it.d_a_m, it.d_b_m, nullptr, 0, 1, 0.1);
+        descs[q].seed = it.noise_seed;
+        const Shape sh = C.shape_for(g.n3, it.mixed != 0);
+        parts[q] = make_part(g.n3, bs, sh.bs == bs ? sh.cpt : 1);
+        offs[q] = total;
+        total += sys_bytes(g, parts[q], pc, it.mixed != 0);
+    }
+    if ((rc = C.bws.ensure(std::max<size_t>(total, 256)))) return rc;
+    Group<double> &G64 = C.b64;
+    Group<float> &G32 = C.b32;
+    G64.sys.clear();
+    G64.sweep.clear();
+    G32.sys.clear();
+    G32.sweep.clear();
+    G64.bs = G32.bs = bs;
+    G64.mixed = false;
+    G32.mixed = true;
+    std::vector<std::pair<int, int>> where((size_t)nsys);  // (precision, row)
+    std::vector<int> ids64, ids32;
+    for (int q = 0; q < nsys; ++q) {
+        char *base = (char *)C.bws.ptr + offs[q];
+        if (items[q].mixed) {
+            G32.sys.emplace_back();
+            G32.sweep.push_back(descs[q].sweep);
+            sys_carve<float>(G32.sys.back(), descs[q], parts[q], base);
+            where[q] = {1, (int)G32.sys.size() - 1};
+            ids32.push_back((int)G32.sys.size() - 1);
+        } else {
+            G64.sys.emplace_back();
+            G64.sweep.push_back(descs[q].sweep);
+            sys_carve<double>(G64.sys.back(), descs[q], parts[q], base);
+            where[q] = {0, (int)G64.sys.size() - 1};
+            ids64.push_back((int)G64.sys.size() - 1);
+        }
+    }
+    int max_it = 1;
+    for (int q = 0; q < nsys; ++q) max_it = std::max(max_it, items[q].itmax);
+    if ((rc = upload_table<double>(G64, s)) || (rc = upload_table<float>(G32, s))) return rc;
+    if ((rc = build_maps<double>(G64, ids64, G64.iter, s)) || (rc = build_maps<float>(G32, ids32, G32.iter, s)))
+        return rc;
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
     (void)hipEventRecord(e0, s);
-    hipLaunchKernelGGL(k_init_state, dim3(1), dim3(1), 0, s, w.st, tol, itmax, 0, 1, 1, delta);
-    const double *noise = opts ? opts->noise : nullptr;
-    const uint64_t seed = opts ? opts->noise_seed : 0;
-    MFIX_DISPATCH_BS(sh.bs, BS, {
-        hipLaunchKernelGGL((k_setup_residual<BS>), grid_of(P), dim3(BS), 0, s, g, P, a, b, (const double *)x, r64,
-                           w.st, w.partials);
-        launch_finish<FIN_SETUP, 1>(w.st, w.partials, P.nblk, s, nullptr);
-        hipLaunchKernelGGL((k_prepare_a32<BS>), grid_of(P), dim3(BS), 0, s, g, P, (const double *)a, A32);
-        hipLaunchKernelGGL((k_shadow<BS, float>), grid_of(P), dim3(BS), 0, s, g, P, (const double *)r64, w.r, w.rt,
-                           w.x, noise, seed, w.st, w.partials);
-        launch_finish<FIN_SHADOW, 1>(w.st, w.partials, P.nblk, s, nullptr);
-    });
-    auto one = [&]() {
-        enqueue_iteration<float>(w, sh, pc, cmethod, s);
-        MFIX_DISPATCH_BS(sh.bs, BS, {
-            hipLaunchKernelGGL((k_reliable_update<BS>), grid_of(P), dim3(BS), 0, s, g, P, (const double *)a,
-                               (const double *)b, (const double *)x, (const double *)xb, x, xb, (const float *)w.x,
-                               w.r, (const float *)w.rt, w.st, w.partials);
-        });
-        launch_finish<FIN_UPDATE, 3>(w.st, w.partials, P.nblk, s, nullptr);
-    };
-    SolverState *h = C.host_state;
-    rc = drive(one, w.st, h, itmax, s);
-    if (!rc && h->x_cur == 1) {
-        hipError_t e = hipMemcpyAsync(x, xb, sizeof(double) * n3, hipMemcpyDeviceToDevice, s);
-        if (e != hipSuccess) rc = hip_fail(e, "hipMemcpyAsync");
-    }
+    enqueue_setup<double>(G64, G64.iter, s);
+    enqueue_setup<float>(G32, G32.iter, s);
+    rc = run_tables(&G64, &G32, max_it, s, C.use_graphs != 0);
     (void)hipEventRecord(e1, s);
     (void)hipEventSynchronize(e1);
     float ms = 0.f;
@@ -512,12 +865,28 @@ int solve_mixed(const DGrid &g, int pc, const char *cmethod, double tol, int itm
     (void)hipEventDestroy(e1);
     for (auto &p : C.prof) p.collect();
     if (rc) return rc;
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return hip_fail(e, "kernel launch");
-    fill_stats(*h, stats);
-    if (stats) stats->device_ms = ms;
-    if (h->msolve_fatal) return MFIX_LEQ_EDGTSV;
-    return 0;
+    if (nsys > kHostStates) return MFIX_LEQ_EINVAL;
+    SolverState *h64 = C.host_state;
+    SolverState *h32 = C.host_state + ids64.size();
+    if ((rc = read_states<double>(G64, ids64, h64, s)) || (rc = read_states<float>(G32, ids32, h32, s))) return rc;
+    int fatal = 0;
+    for (int q = 0; q < nsys; ++q) {
+        const SolverState &h = where[q].first ? h32[where[q].second] : h64[where[q].second];
+        if (where[q].first && h.x_cur == 1) {
+            const Sys<float> &S = G32.sys[where[q].second];
+            hipError_t e = hipMemcpyAsync(S.xa, S.xb, sizeof(double) * S.g.n3, hipMemcpyDeviceToDevice, s);
+            if (e != hipSuccess) return hip_fail(e, "solution copy");
+        }
+        if (stats) {
+            fill_stats(h, &stats[q]);
+            stats[q].device_ms = ms;
+        }
+        fatal |= h.msolve_fatal;
+    }
+    hipError_t e = hipStreamSynchronize(s);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "batch");
+    return fatal ? MFIX_LEQ_EDGTSV : 0;
 }
 
 // ---- standalone kernels of the path -----------------------------------------
@@ -550,34 +919,42 @@ int run_dot(const DGrid &g, const double *x, const double *y, double *d_out, hip
     int rc;
     if ((rc = C.partials.ensure(sizeof(double) * P.nblk * kMaxPartials + 64))) return rc;
     if ((rc = C.state.ensure(sizeof(SolverState)))) return rc;
+    SolverState *st = (SolverState *)C.state.ptr;
+    hipLaunchKernelGGL(k_init_state, dim3(1), dim3(1), 0, s, st);
     MFIX_DISPATCH_BS(sh.bs, BS,
                      hipLaunchKernelGGL((k_dot<BS, double>), grid_of(P), dim3(BS), 0, s, g, P, x, y, g.dot_all ? 0 : 1,
                                         (double *)C.partials.ptr));
-    launch_finish<FIN_DOT, 1>((SolverState *)C.state.ptr, (const double *)C.partials.ptr, P.nblk, s, d_out);
+    hipLaunchKernelGGL((k_finish<FIN_DOT, 1>), dim3((unsigned)finish_blocks(P.nblk)), dim3(kFinishBlock), 0, s, st,
+                       (const double *)C.partials.ptr, P.nblk, d_out);
     return 0;
 }
 
+// Unit scaling alone (LEQ_BICGS0's first step): the set-up kernel of a
+// one-system table with x = 0; the residual it also forms is discarded.
 int run_scale(const DGrid &g, double *a, double *b, int *zero_row, hipStream_t s) {
     Context &C = ctx();
     const int64_t n3 = g.n3;
+    int rc;
+    if ((rc = C.tmp.ensure(sizeof(double) * n3))) return rc;
+    hipError_t e = hipMemsetAsync(C.tmp.ptr, 0, sizeof(double) * n3, s);
+    if (e != hipSuccess) return hip_fail(e, "memset");
+    const SysDesc d = make_desc(g, PC_NONE, "RSRS", 0.0, 1, (double *)C.tmp.ptr, a, b, nullptr, 0, 1, 0.0);
     const Shape sh = C.shape_for(n3, false);
     const Part P = make_part(n3, sh.bs, sh.cpt);
-    int rc;
-    if ((rc = C.partials.ensure(sizeof(double) * P.nblk * kMaxPartials + 64))) return rc;
-    if ((rc = C.state.ensure(sizeof(SolverState)))) return rc;
-    if ((rc = C.ws64.ensure(sizeof(double) * n3 * 2))) return rc;
-    SolverState *st = (SolverState *)C.state.ptr;
-    double *x0 = (double *)C.ws64.ptr, *r = x0 + n3;
-    hipError_t e = hipMemsetAsync(x0, 0, sizeof(double) * n3, s);
-    if (e != hipSuccess) return hip_fail(e, "memset");
-    hipLaunchKernelGGL(k_init_state, dim3(1), dim3(1), 0, s, st, 0.0, 1, 0, 1, 0, 0.0);
+    Group<double> &G = C.g64;
+    if ((rc = C.ws.ensure(sys_bytes(g, P, PC_NONE, false)))) return rc;
+    G.sys.resize(1);
+    G.sweep.assign(1, d.sweep);
+    G.bs = sh.bs;
+    G.mixed = false;
+    sys_carve<double>(G.sys[0], d, P, (char *)C.ws.ptr);
+    if ((rc = upload_table<double>(G, s)) || (rc = build_maps<double>(G, std::vector<int>{0}, G.iter, s))) return rc;
+    const int nids = 1;
+    hipLaunchKernelGGL(k_init_states<double>, dim3(1), dim3(256), 0, s, G.d_sys, G.iter.d_ids, nids);
     MFIX_DISPATCH_BS(sh.bs, BS,
-                     hipLaunchKernelGGL((k_setup_residual<BS>), grid_of(P), dim3(BS), 0, s, g, P, a, b,
-                                        (const double *)x0, r, st, (double *)C.partials.ptr));
-    e = hipMemcpyAsync(C.host_state, st, sizeof(SolverState), hipMemcpyDeviceToHost, s);
-    if (e != hipSuccess) return hip_fail(e, "copy");
-    e = hipStreamSynchronize(s);
-    if (e != hipSuccess) return hip_fail(e, "scale");
+                     hipLaunchKernelGGL((k_setup_residual<BS, double>), dim3((unsigned)G.iter.smap.size()), dim3(BS),
+                                        0, s, G.d_sys, G.iter.d_smap));
+    if ((rc = read_states<double>(G, G.iter.ids, C.host_state, s))) return rc;
     *zero_row = C.host_state->zero_row;
     return 0;
 }

@@ -19,7 +19,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
-from ._lib import Grid, Opts, Stats, check
+from ._lib import BatchItem, Grid, Opts, Stats, check
 
 
 class SolverAbort(RuntimeError):
@@ -229,3 +229,55 @@ def prof_read(which: int) -> dict:
     by = C.c_double()
     check(_lib_().mfix_leq_prof_read(which, C.byref(n), C.byref(ms), C.byref(by)), "prof_read")
     return {"launches": n.value, "total_ms": ms.value, "alg_bytes": by.value}
+
+
+def set_graphs(on: bool) -> None:
+    """Run iterations as captured HIP-graph replays (default) or direct launches."""
+    check(_lib_().mfix_leq_set_graphs(int(on)), "set_graphs")
+
+
+# ---- batched solves -------------------------------------------------------------
+
+@dataclass
+class BatchSystem:
+    """One system of a batch: its grid, solver settings and resident arrays."""
+    imax: int
+    jmax: int
+    kmax: int
+    var: DeviceArray
+    a_m: DeviceArray
+    b_m: DeviceArray
+    pc: str = "DIAG"
+    cmethod: str = "RSRS"
+    tol: float = 1e-10
+    itmax: int = 500
+    mixed: bool = False
+    seed: int = 0
+
+
+def _pad4(s: str) -> bytes:
+    return s.encode().ljust(4)[:4]
+
+
+def batch_solve(systems: list[BatchSystem]) -> list[SolveResult]:
+    """Solve independent systems together: one launch per phase covers all of
+    them (mfix_leq_batch_solve).  Each result equals the system's single
+    solve bit for bit."""
+    n = len(systems)
+    items = (BatchItem * max(n, 1))()
+    for q, s in enumerate(systems):
+        it = items[q]
+        it.imax, it.jmax, it.kmax = s.imax, s.jmax, s.kmax
+        it.pc = _pad4(s.pc)
+        it.cmethod = _pad4(s.cmethod)
+        it.tol = float(s.tol)
+        it.itmax = int(s.itmax)
+        it.mixed = int(bool(s.mixed))
+        it.d_var, it.d_a_m, it.d_b_m = s.var.ptr, s.a_m.ptr, s.b_m.ptr
+        it.noise_seed = int(s.seed)
+    stats = (Stats * max(n, 1))()
+    rc = _lib_().mfix_leq_batch_solve(n, items, stats, None)
+    if rc == -101:
+        raise SolverAbort("preconditioner option not found - check mfix.dat and readme")
+    check(rc, "mfix_leq_batch_solve")
+    return [SolveResult(stats[q].ier, stats[q].as_dict()) for q in range(n)]

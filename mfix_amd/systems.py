@@ -53,3 +53,29 @@ def config_system(cfg: int, sys_index: int = 0, n: int | None = None):
     side = {0: 32, 1: 128, 2: 256}[cfg] if n is None else n
     kind = {0: PRESSURE, 1: MOMENTUM, 2: VOID_PRESSURE}[cfg]
     return gen_system(kind, side, side, side, 1000 * cfg + sys_index)
+
+
+def batch_config(nsys: int = 64, seed: int = 3000, lo: int = 16, hi: int = 128):
+    """The batched workload: sides log-uniform in [lo, hi], half symmetric
+    pressure-correction and half upwind momentum systems, DIAG or LINE per
+    system, each with its own seed.  Returns a list of dicts (no arrays)."""
+    rng = np.random.default_rng(seed)
+    sides = np.rint(np.exp(rng.uniform(np.log(lo), np.log(hi), nsys))).astype(int)
+    kinds = np.array([PRESSURE, MOMENTUM] * ((nsys + 1) // 2))[:nsys]
+    rng.shuffle(kinds)
+    pcs = rng.choice(["DIAG", "LINE"], size=nsys)
+    return [dict(side=int(sides[q]), kind=int(kinds[q]), pc=str(pcs[q]), seed=seed + q) for q in range(nsys)]
+
+
+def serving_requests(rate: float, n: int, seed: int = 4000, median: float = 64.0, sigma: float = 0.5,
+                     lo: int = 24, hi: int = 160):
+    """A Poisson request stream: arrival times (s) at `rate` per second,
+    cube sides lognormal (median, sigma) clipped to [lo, hi], 60 % upwind
+    momentum / 40 % symmetric pressure-correction, TOL 1e-8 or 1e-10."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, n))
+    sides = np.clip(np.rint(median * np.exp(sigma * rng.standard_normal(n))), lo, hi).astype(int)
+    kinds = np.where(rng.uniform(size=n) < 0.6, MOMENTUM, PRESSURE)
+    tols = np.where(rng.uniform(size=n) < 0.5, 1e-8, 1e-10)
+    return [dict(t=float(t[q]), side=int(sides[q]), kind=int(kinds[q]), tol=float(tols[q]), seed=seed + q)
+            for q in range(n)]
