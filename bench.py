@@ -62,6 +62,160 @@ def cpu_baseline(a, b, n, its_per_solve, sample_its, threads):
     }
 
 
+def _bytes_per_cell(pc, mixed):
+    """Fused-schedule algorithmic bytes per cell per iteration (sA, sV, sX =
+    coefficient, Krylov-vector and solution element sizes)."""
+    sa, sv, sx = (4, 4, 8) if mixed else (8, 8, 8)
+    return {"DIAG": 15 * sa + 15 * sv + 2 * sx, "NONE": 14 * sa + 15 * sv + 2 * sx,
+            "LINE": 42 * sa + 30 * sv + 2 * sx}[pc]
+
+
+def bench_batch(args):
+    """64 independent systems, sides log-uniform in [16,128], half symmetric
+    pressure-correction and half upwind momentum, DIAG or LINE per system,
+    solved together by mfix_leq_batch_solve (one launch per phase for all
+    systems, iterations replayed from captured HIP graphs).  A step is one
+    batched solve of all 64 from fresh resident copies of A_M/B_M (the copy
+    refresh is outside the timed call)."""
+    from mfix_amd import leq, systems, _lib
+    cfg = systems.batch_config(64, seed=3000)
+    held = []
+    for c in cfg:
+        side = c["side"]
+        a, b = systems.gen_system(c["kind"], side, side, side, c["seed"])
+        held.append(dict(c=c, a0=leq.DeviceArray.from_host(a), b0=leq.DeviceArray.from_host(b),
+                         da=leq.DeviceArray(a.size), db=leq.DeviceArray(b.size), dx=leq.DeviceArray(b.size),
+                         zeros=np.zeros(b.size), n3=b.size))
+
+    def run(mixed):
+        items = []
+        for h in held:
+            h["da"].copy_from(h["a0"])
+            h["db"].copy_from(h["b0"])
+            h["dx"].upload(h["zeros"])
+            c = h["c"]
+            items.append(leq.BatchSystem(c["side"], c["side"], c["side"], h["dx"], h["da"], h["db"], pc=c["pc"],
+                                         cmethod="RSRS", tol=1e-10, itmax=5000, mixed=mixed, seed=c["seed"]))
+        _lib.check(_lib.load().mfix_leq_sync(), "sync")
+        t = time.perf_counter()
+        res = leq.batch_solve(items)
+        dt = time.perf_counter() - t
+        bad = [q for q, r in enumerate(res) if r.ier != 0]
+        if bad:
+            raise RuntimeError(f"batched systems {bad} did not converge: {[res[q].stats for q in bad]}")
+        return dt, res
+
+    out = {}
+    for mixed in (True, False):
+        for _ in range(args.warmup):
+            run(mixed)
+        times, alg = [], []
+        for _ in range(args.steps if mixed else max(1, args.fp64_steps)):
+            dt, res = run(mixed)
+            times.append(dt)
+            alg.append(sum(r.stats["iterations"] * h["n3"] * _bytes_per_cell(h["c"]["pc"], mixed)
+                           for r, h in zip(res, held)))
+        total = sum(times)
+        out[mixed] = dict(value=len(held) * len(times) / total, ms=1e3 * total / len(times),
+                          alg_rate=sum(alg) / total, iters=[r.stats["iterations"] for r in res],
+                          true_rel=max((r.stats["true_rnorm"] / r.stats["rnorm0"]) for r in res) if mixed else None,
+                          lat=times)
+    m, f = out[True], out[False]
+    sides = [h["c"]["side"] for h in held]
+    line = {
+        "metric": "solves/s (batched, FP32-refined, 1e-10 relative residual)",
+        "value": m["value"], "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": m["ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64", "data": "synthetic",
+        "config": {
+            "workload": "64 systems, sides log-uniform 16..128 (seed 3000), 50% symmetric pressure / 50% upwind "
+                        "momentum, DIAG or LINE(RSRS) per system, TOL=1e-10, one batched solve per step",
+            "sides_min_median_max": [min(sides), float(np.median(sides)), max(sides)],
+            "total_cells": int(sum(h["n3"] for h in held)),
+            "iterations_sum": int(sum(m["iters"])),
+            "fp64_solves_per_s": f["value"], "fp64_iterations_sum": int(sum(f["iters"])),
+            "speedup_vs_fp64": m["value"] / f["value"],
+            "solve_alg_GBps": m["alg_rate"] / 1e9, "solve_roofline_frac": m["alg_rate"] / HBM_PEAK,
+            "fp64_alg_GBps": f["alg_rate"] / 1e9, "fp64_roofline_frac": f["alg_rate"] / HBM_PEAK,
+            "max_true_rel_residual": m["true_rel"],
+        },
+        "roofline": {"bound": "hbm", "kernel": "whole batched solve (fused-schedule bytes)",
+                     "achieved": m["alg_rate"] / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": m["alg_rate"] / HBM_PEAK, "traffic": None},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line))
+    return 0
+
+
+def bench_serve(args):
+    """Poisson request streams through the continuous-batching server: sides
+    lognormal (median 64, sigma 0.5) clipped to [24,160], 60% upwind momentum
+    (LINE) / 40% symmetric pressure (DIAG), TOL 1e-8 or 1e-10.  Latency is
+    arrival -> solution in the caller's host VAR; throughput is completed
+    requests over the span from the first arrival to the last completion."""
+    from mfix_amd import leq, systems
+    from mfix_amd.serve import Server
+    rates = [float(r) for r in args.rates.split(",") if r]
+    nreq = args.requests
+    proto = systems.serving_requests(rates[0], nreq, seed=4000)
+    arrays = []
+    for r in proto:
+        side = r["side"]
+        a, b = systems.gen_system(r["kind"], side, side, side, r["seed"])
+        arrays.append((a, b))
+    results = []
+    for mixed in (True, False):
+        for rate in rates:
+            reqs = systems.serving_requests(rate, nreq, seed=4000)
+            with Server(64 << 30, 64) as srv:
+                live = {}
+                done_t = {}
+                nxt = 0
+                t0 = time.perf_counter()
+                while len(done_t) < nreq:
+                    now = time.perf_counter() - t0
+                    while nxt < nreq and reqs[nxt]["t"] <= now:
+                        r = reqs[nxt]
+                        a, b = arrays[nxt]
+                        pc = "LINE" if r["kind"] == systems.MOMENTUM else "DIAG"
+                        rid = srv.submit(a, b, np.zeros(b.size), r["side"], r["side"], r["side"],
+                                         pc=pc, tol=r["tol"], itmax=5000, mixed=mixed, seed=r["seed"])
+                        live[rid] = nxt
+                        nxt += 1
+                    if srv.pending():
+                        srv.step()
+                    elif nxt < nreq:
+                        time.sleep(max(0.0, min(0.001, reqs[nxt]["t"] - (time.perf_counter() - t0))))
+                    for rid in list(live):
+                        d = srv.poll(rid)
+                        if d is not None:
+                            if d.ier != 0:
+                                raise RuntimeError(f"request {live[rid]} failed: {d.stats}")
+                            done_t[live.pop(rid)] = time.perf_counter() - t0
+                lat = [done_t[q] - reqs[q]["t"] for q in range(nreq)]
+                span = max(done_t.values()) - reqs[0]["t"]
+                results.append(dict(mixed=mixed, rate=rate, p50_ms=1e3 * _percentile(lat, 50),
+                                    p99_ms=1e3 * _percentile(lat, 99), solves_per_s=nreq / span))
+                print(f"serve mixed={mixed} rate={rate:g}/s p50={results[-1]['p50_ms']:.1f} ms "
+                      f"p99={results[-1]['p99_ms']:.1f} ms {results[-1]['solves_per_s']:.1f} solves/s",
+                      file=sys.stderr, flush=True)
+    top = max((r for r in results if r["mixed"]), key=lambda r: r["rate"])
+    line = {
+        "metric": "p50/p99 latency and solves/s under Poisson load (continuous batching)",
+        "value": top["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": nreq, "warmup": 0,
+        "ms_per_step": 1e3 / top["solves_per_s"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": f"{nreq} requests per rate, sides lognormal(64, 0.5) clipped to [24,160], 60% "
+                               "upwind LINE / 40% pressure DIAG, TOL 1e-8|1e-10, HBM budget 64 GiB, 64 slots",
+                   "p50_ms": top["p50_ms"], "p99_ms": top["p99_ms"], "rate": top["rate"], "sweep": results},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line))
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,6 +225,14 @@ def main(argv=None):
     ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--cpu-iters", type=int, default=6, help="reference iterations timed for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--workload", choices=["mixed", "batch", "serve"], default="mixed",
+                    help="mixed: the 256^3 solve (headline); batch: 64 systems 16^3..128^3 in one batched "
+                         "solve; serve: Poisson request streams through the continuous-batching server")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run the plain copy kernel 3x on 512 MiB (HBM counter calibration)")
+    ap.add_argument("--rates", default="50,200,1000,2000", help="serve: request rates (1/s)")
+    ap.add_argument("--requests", type=int, default=120, help="serve: requests per rate")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this solver has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -83,6 +245,17 @@ def main(argv=None):
         from mfix_amd import build
         build.build()
     leq.init(0)
+    if args.copy_calibration:
+        src = leq.DeviceArray(64 << 20)
+        dst = leq.DeviceArray(64 << 20)
+        for _ in range(3):
+            _lib.check(_lib.load().mfix_leq_copy_kernel_dev(dst.ptr, src.ptr, 64 << 20), "copy kernel")
+        src.free()
+        dst.free()
+    if args.workload == "batch":
+        return bench_batch(args)
+    if args.workload == "serve":
+        return bench_serve(args)
     n = args.n
     t0 = time.time()
     a, b = systems.gen_system(systems.VOID_PRESSURE, n, n, n, 2000)
@@ -91,7 +264,7 @@ def main(argv=None):
     leq.grid_init(n, n, n)
     blk = C.c_int()
     lpb = C.c_int()
-    for mixed in (1, 0):
+    for mixed in ((1, 0) if not args.no_tune else ()):
         _lib.check(_lib.load().mfix_leq_autotune(n, n, n, mixed, C.byref(blk), C.byref(lpb)), "autotune")
     a0 = leq.DeviceArray.from_host(a)
     b0 = leq.DeviceArray.from_host(b)

@@ -230,7 +230,7 @@ int mfix_leq_msolve_dev(const char *pc, const char *cmethod, const double *d_a_m
         run_diag(g, d_a_m, d_r, d_z, s);
         return sync_status(s, "msolve");
     }
-    if ((rc = C.tmp.ensure(sizeof(double) * g.n3 * 4))) return rc;
+    if ((rc = C.tmp.ensure(sizeof(double) * (size_t)line_scratch_elems(g)))) return rc;
     if ((rc = C.misc.ensure(sizeof(int) * 4 + sizeof(double) * 8))) return rc;
     int *flag = (int *)C.misc.ptr;
     e = hipMemsetAsync(flag, 0, sizeof(int), s);

@@ -44,7 +44,8 @@ struct SolverState {
     int updates;            // mixed: reliable updates performed
     int mixed;              // 1 = inner solve of the mixed-precision path
     int final_update;       // mixed: the pending update ends the solve
-    int y_reset;            // mixed: the FP32 correction accumulator restarts from 0
+    int y_reset;            // mixed: the FP64 correction accumulator restarts from 0
+    int discard_y;          // mixed: a non-finite scalar appeared; the next update drops y
     int x_cur;              // mixed: which FP64 solution buffer is current
     int breakdown;          // mixed: rho = 0 seen before the final update
     int restart;            // mixed: restart the recurrence (p = r, Rtilde = r) at the next iteration
@@ -202,6 +203,20 @@ __device__ __forceinline__ void fin_apply(SolverState *st, const double (&tot)[N
     } else if (F == FIN_XR_MIXED) {
         S.y_reset = 0;
         S.restart = 0;
+        if (!isfinite(tot[0]) || !isfinite(tot[1]) || !isfinite(S.alpha) || !isfinite(S.omega)) {
+            // an FP32 breakdown (a zero or overflowing denominator) reached
+            // the recurrence: restart it from the last FP64 solution
+            S.need_update = 1;
+            S.discard_y = 1;
+            S.iter_var += 1;
+            if (S.i >= S.itmax) {
+                S.final_update = 1;
+                S.exit_reason = EXIT_ITMAX;
+            } else {
+                S.i += 1;
+            }
+            return;
+        }
         if (S.s_exit) {
             S.need_update = 1;
             S.final_update = 1;
@@ -232,12 +247,20 @@ __device__ __forceinline__ void fin_apply(SolverState *st, const double (&tot)[N
         S.rho_prev = S.rho;
         S.rho = rho_new;
     } else if (F == FIN_UPDATE) {
+        const bool discarded = S.discard_y;
+        S.discard_y = 0;
         S.x_cur = 1 - S.x_cur;
         S.y_reset = 1;
         S.need_update = 0;
         S.updates += 1;
         const double prev_true = S.upd_rnorm;
         S.true_rnorm = sqrt(tot[0]);
+        if (!isfinite(S.true_rnorm)) {
+            S.ier = -2;
+            S.exit_reason = EXIT_RHO;
+            S.done = 1;
+            return;
+        }
         S.upd_rnorm = S.true_rnorm;
         S.rnorm_max = S.true_rnorm;
         if (S.true_rnorm <= S.tol * S.rnorm0) {
@@ -260,7 +283,7 @@ __device__ __forceinline__ void fin_apply(SolverState *st, const double (&tot)[N
         // updates make progress; restart it (p = r, Rtilde = r, rho = <r,r>)
         // when the true residual grew since the last update or the FP32
         // recurrence drifted more than 10x away from it.
-        const bool drift = S.true_rnorm > prev_true || S.true_rnorm > 10.0 * S.rnorm;
+        const bool drift = discarded || S.true_rnorm > prev_true || S.true_rnorm > 10.0 * S.rnorm;
         const double rho_cont = tot[1];
         if (drift || rho_cont == 0.0) {
             S.restart = 1;
@@ -284,9 +307,16 @@ __device__ __forceinline__ void fin_apply(SolverState *st, const double (&tot)[N
 // ---- LINE preconditioner (LEQ_MSOLVE, leqsol_mod.f:286-598) ---------------
 //
 // One thread solves one line with the DGTSV algorithm of model/DGTSV.f:107-163
-// (elimination with partial pivoting) and :233-246 (back solve), working in
-// four scratch planes indexed like the cells of the line, so the result is
-// the reference's bit for bit.  Lines of one phase are independent (red-black
+// (elimination with partial pivoting) and :233-246 (back solve), in the same
+// operation order, so the result is the reference's bit for bit.  The
+// elimination carries the two active rows in registers and reads each row's
+// inputs (three tridiagonal coefficients and the right-hand side formed from
+// the cross-line neighbours, LEQ_IKSWEEP leqsol_mod.f:891-902 and its JK/IJ
+// twins) once, as it reaches them; only the finished rows (diagonal, first
+// and second super-diagonal, right-hand side) go to scratch, each written
+// once and read once by the back solve.  The thread's scratch rows are
+// contiguous (one cache line serves 16 FP64 rows), so no step waits on a
+// store it just made.  Lines of one phase are independent (red-black
 // colours, anti-diagonal wavefronts), so a phase is one launch.
 
 enum LineAxis : int { AX_J = 0, AX_I = 1, AX_K = 2 };
@@ -317,6 +347,13 @@ __host__ __device__ inline LineGeom line_geom(const DGrid &g, int axis) {
     return L;
 }
 
+// Scratch elements a line launch needs: 4 arrays of (lanes x line length);
+// every launch here uses at most n3/n + 64 lanes of at most 64 threads.
+__host__ __device__ inline int64_t line_scratch_elems(const DGrid &g) {
+    const int nmax = g.ni > g.nj ? (g.ni > g.nk ? g.ni : g.nk) : (g.nj > g.nk ? g.nj : g.nk);
+    return 4 * (g.n3 + 64 * (int64_t)nmax);
+}
+
 // Position p (1-based) of the reference's red-black enumeration of a 2-D
 // index set (leqsol_mod.f:404-418) -> 1-based (fast, slow).
 __device__ __forceinline__ void rb_position(int p, int s1, int *f, int *s) {
@@ -327,115 +364,138 @@ __device__ __forceinline__ void rb_position(int p, int s1, int *f, int *s) {
     *s = slow;
 }
 
+// Inputs of row q of a line: DL (coupling to row q-1, CC; 0 at q = 0),
+// D, DU (coupling to row q+1, EE; 0 at the last row) and the right-hand
+// side BB = B - cross-line neighbour terms in the reference's order.
 template <typename T>
-__device__ void solve_line(const DGrid &g, const LineGeom &L, int axis, int64_t c0, const T *__restrict__ a,
-                           const T *__restrict__ bm, T *__restrict__ var, T *__restrict__ sdl,
-                           T *__restrict__ sd, T *__restrict__ sdu, T *__restrict__ sb, int fatal,
-                           int *fatal_flag) {
+struct LineRow {
+    T dl, d, du, b;
+};
+
+template <typename T>
+__device__ __forceinline__ LineRow<T> line_row(const DGrid &g, const LineGeom &L, int axis, int64_t c0, int q,
+                                               const T *__restrict__ a, const T *__restrict__ bm,
+                                               const T *__restrict__ var) {
     const int64_t n3 = g.n3;
-    const int n = L.n;
     const int pd = DiagPlane<T>::value;
-    // plane indices: lower/upper along the line; the two cross axes
     int plo, pup, x1m, x1p, x2m, x2p;
     if (axis == AX_J) { plo = 1; pup = 5; x1m = 2; x1p = 4; x2m = 0; x2p = 6; }
     else if (axis == AX_I) { plo = 2; pup = 4; x1m = 1; x1p = 5; x2m = 0; x2p = 6; }
     else { plo = 0; pup = 6; x1m = 1; x1p = 5; x2m = 2; x2p = 4; }
-    // build DD, CC, EE, BB (LEQ_IKSWEEP leqsol_mod.f:891-902 and its JK/IJ twins)
-    for (int q = 0; q < n; ++q) {
-        const int64_t c = c0 + (int64_t)q * L.stride;
-        const Nb nb = neighbours(g, c);
-        int64_t m1, p1, m2, p2;
-        if (axis == AX_J) { m1 = nb.im; p1 = nb.ip; m2 = nb.km; p2 = nb.kp; }
-        else if (axis == AX_I) { m1 = nb.jm; p1 = nb.jp; m2 = nb.km; p2 = nb.kp; }
-        else { m1 = nb.jm; p1 = nb.jp; m2 = nb.im; p2 = nb.ip; }
-        T bb;
-        if (axis == AX_J && !g.do_k)
-            bb = bm[c] - a[x1m * n3 + c] * var[m1] - a[x1p * n3 + c] * var[p1];
-        else
-            bb = bm[c] - a[x1m * n3 + c] * var[m1] - a[x1p * n3 + c] * var[p1] - a[x2m * n3 + c] * var[m2] -
-                 a[x2p * n3 + c] * var[p2];
-        sd[c] = a[pd * n3 + c];
-        sdl[c] = (q == 0) ? (T)0 : a[plo * n3 + c];     // CC(NSTART) = 0
-        sdu[c] = (q == n - 1) ? (T)0 : a[pup * n3 + c]; // EE(NEND) = 0
-        sb[c] = bb;
-    }
-    // DGTSV.f:107-163; DL(i) is CC of row i+1, stored at that row's cell
-#define AT(q) (c0 + (int64_t)(q) * L.stride)
+    const int64_t c = c0 + (int64_t)q * L.stride;
+    const Nb nb = neighbours(g, c);
+    int64_t m1, p1, m2, p2;
+    if (axis == AX_J) { m1 = nb.im; p1 = nb.ip; m2 = nb.km; p2 = nb.kp; }
+    else if (axis == AX_I) { m1 = nb.jm; p1 = nb.jp; m2 = nb.km; p2 = nb.kp; }
+    else { m1 = nb.jm; p1 = nb.jp; m2 = nb.im; p2 = nb.ip; }
+    LineRow<T> r;
+    if (axis == AX_J && !g.do_k)
+        r.b = bm[c] - a[x1m * n3 + c] * var[m1] - a[x1p * n3 + c] * var[p1];
+    else
+        r.b = bm[c] - a[x1m * n3 + c] * var[m1] - a[x1p * n3 + c] * var[p1] - a[x2m * n3 + c] * var[m2] -
+              a[x2p * n3 + c] * var[p2];
+    r.d = a[pd * n3 + c];
+    r.dl = q == 0 ? (T)0 : a[plo * n3 + c];
+    r.du = q == L.n - 1 ? (T)0 : a[pup * n3 + c];
+    return r;
+}
+
+// Solve the line starting at cell c0 in place of var.  `scratch` holds 4
+// arrays of nl x L.n elements; this thread uses slot `lane` of each.
+template <typename T>
+__device__ void solve_line(const DGrid &g, const LineGeom &L, int axis, int64_t c0, const T *__restrict__ a,
+                           const T *__restrict__ bm, T *__restrict__ var, T *__restrict__ scratch, int64_t lane,
+                           int64_t nl, int fatal, int *fatal_flag) {
+    const int n = L.n;
+    const int64_t plane = nl * n;
+    T *__restrict__ sD = scratch + lane * n;       // final diagonal
+    T *__restrict__ sU = sD + plane;               // first super-diagonal
+    T *__restrict__ sU2 = sD + 2 * plane;          // second super-diagonal (pivoting fill, DL(i) after :127)
+    T *__restrict__ sB = sD + 3 * plane;           // right-hand side
+    LineRow<T> r0 = line_row<T>(g, L, axis, c0, 0, a, bm, var);
+    T d0 = r0.d, du0 = r0.du, b0 = r0.b;
     int info = 0;
-    for (int q = 0; q < n - 2 && !info; ++q) {
-        const T dq = sd[AT(q)], dlq = sdl[AT(q + 1)];
-        if (fabs((double)dq) >= fabs((double)dlq)) {
-            if (dq != (T)0) {
-                const T fact = dlq / dq;
-                sd[AT(q + 1)] = sd[AT(q + 1)] - fact * sdu[AT(q)];
-                sb[AT(q + 1)] = sb[AT(q + 1)] - fact * sb[AT(q)];
-            } else {
-                info = q + 1;
+    LineRow<T> nx;
+    if (n > 1) nx = line_row<T>(g, L, axis, c0, 1, a, bm, var);
+    for (int i = 0; i < n - 1; ++i) {
+        // row i+1 (its inputs were read one step ahead)
+        const T dl1 = nx.dl;
+        T d1 = nx.d, du1 = nx.du, b1 = nx.b;
+        if (i + 2 < n) nx = line_row<T>(g, L, axis, c0, i + 2, a, bm, var);
+        T fd, fu, fu2, fb;
+        if (fabs((double)d0) >= fabs((double)dl1)) {
+            // no row interchange (DGTSV.f:111-122 / :139-147)
+            if (d0 == (T)0) {
+                info = i + 1;
                 break;
             }
-            sdl[AT(q + 1)] = (T)0;
+            const T fact = dl1 / d0;
+            d1 = d1 - fact * du0;
+            b1 = b1 - fact * b0;
+            fd = d0;
+            fu = du0;
+            fu2 = (T)0;
+            fb = b0;
         } else {
-            const T fact = dq / dlq;
-            sd[AT(q)] = dlq;
-            T temp = sd[AT(q + 1)];
-            sd[AT(q + 1)] = sdu[AT(q)] - fact * temp;
-            sdl[AT(q + 1)] = sdu[AT(q + 1)];
-            sdu[AT(q + 1)] = -fact * sdl[AT(q + 1)];
-            sdu[AT(q)] = temp;
-            temp = sb[AT(q)];
-            sb[AT(q)] = sb[AT(q + 1)];
-            sb[AT(q + 1)] = temp - fact * sb[AT(q + 1)];
-        }
-    }
-    if (!info && n > 1) {
-        const int q = n - 2;
-        const T dq = sd[AT(q)], dlq = sdl[AT(q + 1)];
-        if (fabs((double)dq) >= fabs((double)dlq)) {
-            if (dq != (T)0) {
-                const T fact = dlq / dq;
-                sd[AT(q + 1)] = sd[AT(q + 1)] - fact * sdu[AT(q)];
-                sb[AT(q + 1)] = sb[AT(q + 1)] - fact * sb[AT(q)];
+            // interchange rows i and i+1 (DGTSV.f:123-136 / :148-158)
+            const T fact = d0 / dl1;
+            fd = dl1;
+            T temp = d1;
+            d1 = du0 - fact * temp;
+            if (i < n - 2) {
+                fu2 = du1;
+                du1 = -fact * fu2;
             } else {
-                info = q + 1;
+                fu2 = (T)0;  // the last interchange leaves no fill (never read)
             }
-        } else {
-            const T fact = dq / dlq;
-            sd[AT(q)] = dlq;
-            T temp = sd[AT(q + 1)];
-            sd[AT(q + 1)] = sdu[AT(q)] - fact * temp;
-            sdu[AT(q)] = temp;
-            temp = sb[AT(q)];
-            sb[AT(q)] = sb[AT(q + 1)];
-            sb[AT(q + 1)] = temp - fact * sb[AT(q + 1)];
+            fu = temp;
+            temp = b0;
+            fb = b1;
+            b1 = temp - fact * b1;
         }
+        sD[i] = fd;
+        sU[i] = fu;
+        sU2[i] = fu2;
+        sB[i] = fb;
+        d0 = d1;
+        du0 = du1;
+        b0 = b1;
     }
-    if (!info && sd[AT(n - 1)] == (T)0) info = n;
+    if (!info && d0 == (T)0) info = n;  // DGTSV.f:160-163
     if (info) {
         if (fatal) atomicOr(fatal_flag, 1);
         return;  // the line keeps its current values (leqsol_mod.f:910-914)
     }
-    // back solve, DGTSV.f:233-246 (DL(i) here is the fill-in of row i+1)
-    sb[AT(n - 1)] = sb[AT(n - 1)] / sd[AT(n - 1)];
-    if (n > 1) sb[AT(n - 2)] = (sb[AT(n - 2)] - sdu[AT(n - 2)] * sb[AT(n - 1)]) / sd[AT(n - 2)];
-    for (int q = n - 3; q >= 0; --q)
-        sb[AT(q)] = (sb[AT(q)] - sdu[AT(q)] * sb[AT(q + 1)] - sdl[AT(q + 1)] * sb[AT(q + 2)]) / sd[AT(q)];
-    for (int q = 0; q < n; ++q) var[AT(q)] = sb[AT(q)];
-#undef AT
+    // back solve, DGTSV.f:233-246
+    const int64_t st = L.stride;
+    T x1 = b0 / d0;
+    var[c0 + (int64_t)(n - 1) * st] = x1;
+    if (n == 1) return;
+    T x0 = (sB[n - 2] - sU[n - 2] * x1) / sD[n - 2];
+    var[c0 + (int64_t)(n - 2) * st] = x0;
+    T x2 = x1;
+    x1 = x0;
+    for (int q = n - 3; q >= 0; --q) {
+        const T x = (sB[q] - sU[q] * x1 - sU2[q] * x2) / sD[q];
+        var[c0 + (int64_t)q * st] = x;
+        x2 = x1;
+        x1 = x;
+    }
 }
 
 // One phase of line solves.  `st` (nullable) makes the launch a no-op once
 // the solve is done, or, with skip_on_sexit, after the S-norm exit.
 template <typename T>
 __global__ void __launch_bounds__(64) k_line_phase(DGrid g, LinePhase ph, const T *__restrict__ a,
-                                                   const T *__restrict__ bm, T *__restrict__ var, T *sdl, T *sd,
-                                                   T *sdu, T *sb, const SolverState *st, int skip_on_sexit,
-                                                   int *fatal_flag) {
+                                                   const T *__restrict__ bm, T *__restrict__ var, T *scratch,
+                                                   const SolverState *st, int skip_on_sexit, int *fatal_flag) {
     if (st) {
         if (st->done) return;
         if (skip_on_sexit && st->s_exit) return;
     }
     const LineGeom L = line_geom(g, ph.axis);
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nl = (int64_t)gridDim.x * blockDim.x;
     if (ph.kind == PH_SEQ) {
         // reference enumeration of one red-black pass when the colouring is
         // not proper (odd fast size): one thread walks the lines in order
@@ -445,7 +505,7 @@ __global__ void __launch_bounds__(64) k_line_phase(DGrid g, LinePhase ph, const 
             int f, s;
             rb_position(p, ph.seq_s1, &f, &s);
             const int64_t c0 = (int64_t)(f - 1) * L.su + (int64_t)(s - 1) * L.sv;
-            solve_line<T>(g, L, ph.axis, c0, a, bm, var, sdl, sd, sdu, sb, ph.fatal, fatal_flag);
+            solve_line<T>(g, L, ph.axis, c0, a, bm, var, scratch, 0, nl, ph.fatal, fatal_flag);
         }
         return;
     }
@@ -464,7 +524,7 @@ __global__ void __launch_bounds__(64) k_line_phase(DGrid g, LinePhase ph, const 
         v = d - u;
     }
     const int64_t c0 = (int64_t)u * L.su + (int64_t)v * L.sv;
-    solve_line<T>(g, L, ph.axis, c0, a, bm, var, sdl, sd, sdu, sb, ph.fatal, fatal_flag);
+    solve_line<T>(g, L, ph.axis, c0, a, bm, var, scratch, tid, nl, ph.fatal, fatal_flag);
 }
 
 // Lines of one red-black colour along J, numbered densely: line t of the

@@ -35,7 +35,8 @@ struct Sys {
     int itmax, minimize, icheck, mixed;
     double tol, delta;
     const T *a;            // reference layout (FP64) or difference layout (FP32)
-    T *x, *r, *rt, *p, *s, *t, *v, *phat, *shat;
+    T *x, *r, *rt, *p, *s, *t, *v, *phat, *shat;  // x: VAR (FP64 solve only)
+    double *y;             // mixed: FP64 correction accumulator since the last reliable update
     T *line_scratch;       // 4 planes of DGTSV scratch (LINE)
     SolverState *st;
     double *partials;      // kMaxPartials * P.nblk
@@ -129,7 +130,7 @@ __global__ void __launch_bounds__(BS) k_shadow(const Sys<T> *__restrict__ sys, c
         const T rT = (T)rc;
         if (mixed) {
             S.r[c] = rT;
-            S.x[c] = (T)0;
+            S.y[c] = 0.0;
         }
         if (g.dot_all || is_interior(g, c)) v[0] += (double)t * (double)rT;
     }
@@ -228,8 +229,10 @@ __global__ void __launch_bounds__(BS) k_s_update(const Sys<T> *__restrict__ sys,
 
 // Var += alpha*Phat + omega*Shat; R = S - omega*T (leq_bicgs.f:432-436);
 // ||R||^2 (:440) and the next <Rtilde,R> (:318).  After the S-norm exit only
-// Var += alpha*Phat is applied (:374-376).  In the mixed path x is the FP32
-// correction accumulator, restarted at 0 after a reliable update.
+// Var += alpha*Phat is applied (:374-376).  In the mixed path the update goes
+// to the FP64 correction accumulator y (restarted at 0 after a reliable
+// update) with the same FP32-rounded Phat/Shat the stencil used: keeping y
+// in FP64 stops its rounding from polluting every reliable update.
 template <int BS, typename T>
 __global__ void __launch_bounds__(BS) k_xr_update(const Sys<T> *__restrict__ sys, const int2 *__restrict__ map) {
     __shared__ double lds[(BS / 64) * 2];
@@ -245,12 +248,14 @@ __global__ void __launch_bounds__(BS) k_xr_update(const Sys<T> *__restrict__ sys
     const T *__restrict__ sh = S.pc == PC_LINE ? S.shat : S.s;
     const T *__restrict__ a0 = S.a + (int64_t)DiagPlane<T>::value * g.n3;
     T *__restrict__ x = S.x;
+    double *__restrict__ y = S.y;
+    const bool mixed = S.mixed;
     const T *__restrict__ s = S.s;
     const T *__restrict__ t = S.t;
     T *__restrict__ r = S.r;
     const T *__restrict__ rt = S.rt;
     FOR_CELLS(BS, P, blk, c) {
-        const double x0 = yr ? 0.0 : (double)x[c];
+        const double x0 = mixed ? (yr ? 0.0 : y[c]) : (double)x[c];
         double phv, d0 = 1.0;
         if (div) {
             d0 = (double)a0[c];
@@ -259,10 +264,12 @@ __global__ void __launch_bounds__(BS) k_xr_update(const Sys<T> *__restrict__ sys
             phv = (double)ph[c];
         }
         if (sexit) {
-            x[c] = (T)(x0 + alpha * phv);
+            if (mixed) y[c] = x0 + alpha * phv;
+            else x[c] = (T)(x0 + alpha * phv);
         } else {
             const double shv = div ? (double)(T)((double)sh[c] / d0) : (double)sh[c];
-            x[c] = (T)(x0 + alpha * phv + omega * shv);
+            if (mixed) y[c] = x0 + alpha * phv + omega * shv;
+            else x[c] = (T)(x0 + alpha * phv + omega * shv);
             const T rv = (T)((double)s[c] - omega * (double)t[c]);
             r[c] = rv;
             if (g.dot_all || is_interior(g, c)) {
@@ -276,7 +283,8 @@ __global__ void __launch_bounds__(BS) k_xr_update(const Sys<T> *__restrict__ sys
 
 // Reliable update (mixed path): xo = xi + y, r64 = b - A64*(xi + y) with the
 // neighbours' xi + y formed on the fly (xi is read-only here, xo is the other
-// buffer), r = (float)r64; partial ||r64||^2, <Rtilde, r> and <r, r>.
+// buffer), r = (float)r64; partial ||r64||^2, <Rtilde, r> and <r, r>.  After
+// a breakdown (discard_y) y is dropped: xo = xi.
 template <int BS>
 __global__ void __launch_bounds__(BS) k_reliable_update(const Sys<float> *__restrict__ sys,
                                                         const int2 *__restrict__ map) {
@@ -294,9 +302,9 @@ __global__ void __launch_bounds__(BS) k_reliable_update(const Sys<float> *__rest
     double acc[3] = {0.0, 0.0, 0.0};
     struct XY {
         const double *__restrict__ xi;
-        const float *__restrict__ y;
-        __device__ __forceinline__ double operator()(int64_t q) const { return xi[q] + (double)y[q]; }
-    } X{xi, S.x};
+        const double *__restrict__ y;
+        __device__ __forceinline__ double operator()(int64_t q) const { return y ? xi[q] + y[q] : xi[q]; }
+    } X{xi, st->discard_y ? nullptr : S.y};
     FOR_CELLS(BS, P, blk, c) {
         const Nb n = neighbours_fast(g, c);
         const double rr = b[c] - stencil_row<double>(g, a, X, c, n);
@@ -368,12 +376,13 @@ __global__ void __launch_bounds__(64) k_line_color_sys(const Sys<T> *__restrict_
     const DGrid &g = S.g;
     const LineGeom L = line_geom(g, AX_J);
     int u, v;
-    if (!color_line(L.nu, L.nv, g.do_k ? 1 : 0, param, (int64_t)e.y * 64 + threadIdx.x, u, v)) return;
+    const int64_t lane = (int64_t)e.y * 64 + threadIdx.x;
+    if (!color_line(L.nu, L.nv, g.do_k ? 1 : 0, param, lane, u, v)) return;
     const int64_t c0 = (int64_t)u * L.su + (int64_t)v * L.sv;
-    T *sc = S.line_scratch;
-    const int64_t n3 = g.n3;
-    solve_line<T>(g, L, AX_J, c0, S.a, which == 0 ? S.p : S.s, which == 0 ? S.phat : S.shat, sc, sc + n3,
-                  sc + 2 * n3, sc + 3 * n3, 0, nullptr);
+    // lanes of this system: 64 per block, blocks for the larger colour
+    const int64_t nl = (((int64_t)L.nu * L.nv + 1) / 2 + 63) / 64 * 64;
+    solve_line<T>(g, L, AX_J, c0, S.a, which == 0 ? S.p : S.s, which == 0 ? S.phat : S.shat, S.line_scratch, lane,
+                  nl, 0, nullptr);
 }
 
 // Number of the listed systems still running.

@@ -211,7 +211,6 @@ int launch_msolve_line(const DGrid &g, const Shape &sh, const char *cmethod, con
                        T *scratch, const SolverState *st, int skip_on_sexit, int *fatal_flag, hipStream_t s,
                        bool copy_first) {
     const int64_t n3 = g.n3;
-    T *sdl = scratch, *sd = scratch + n3, *sdu = scratch + 2 * n3, *sb = scratch + 3 * n3;
     Prof &pf = ctx().prof[1];
     if (copy_first) {
         const Part P = make_part(n3, sh.bs, sh.cpt);
@@ -219,7 +218,9 @@ int launch_msolve_line(const DGrid &g, const Shape &sh, const char *cmethod, con
                          hipLaunchKernelGGL((k_copy_if<BS, T>), grid_of(P), dim3(BS), 0, s, P, r, z, st,
                                             skip_on_sexit));
     }
-    const int lpb = sh.lines_per_block > 0 ? sh.lines_per_block : 64;
+    // at most 64 lines per block: the kernel's launch bound and the scratch
+    // size (line_scratch_elems) assume it
+    const int lpb = sh.lines_per_block > 0 && sh.lines_per_block < 64 ? sh.lines_per_block : 64;
     auto run = [&](const LinePhase &ph, int64_t nthreads) {
         const unsigned blocks = (unsigned)((nthreads + lpb - 1) / lpb);
         const LineGeom L = line_geom(g, ph.axis);
@@ -229,8 +230,8 @@ int launch_msolve_line(const DGrid &g, const Shape &sh, const char *cmethod, con
         const double cells = (double)std::min<int64_t>(nthreads, (int64_t)L.nu * L.nv) * L.n *
                              (ph.kind == PH_COLOR ? 0.5 : 1.0);
         pf.begin(s);
-        hipLaunchKernelGGL(k_line_phase<T>, dim3(blocks ? blocks : 1), dim3(lpb), 0, s, g, ph, a, r, z, sdl, sd, sdu,
-                           sb, st, skip_on_sexit, fatal_flag);
+        hipLaunchKernelGGL(k_line_phase<T>, dim3(blocks ? blocks : 1), dim3(lpb), 0, s, g, ph, a, r, z, scratch, st,
+                           skip_on_sexit, fatal_flag);
         pf.end(s, cells * (7.0 * sizeof(T) + 6.0 * sizeof(T)));
     };
     for (int it = 0; it < 4; ++it) {
@@ -302,17 +303,19 @@ double alg_bytes_per_cell(int pc, int sa, int sv, int sx) {
 
 static inline size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
-// FP64: R, Rtilde, P, S, T, V (+ Phat, Shat and 4 DGTSV scratch planes for
-// LINE).  Mixed: the FP32 r, rt, p, s, t, v, y (+ LINE arrays), the 8-plane
-// FP32 matrix, the FP64 residual and second solution buffer.  Both: the
+// FP64: R, Rtilde, P, S, T, V (+ Phat, Shat and the DGTSV scratch for LINE).
+// Mixed: the FP32 r, rt, p, s, t, v (+ LINE arrays), the FP64 correction
+// accumulator y, the 8-plane FP32 matrix, the FP64 residual and second
+// solution buffer.  Both: the
 // partial sums and the solver state.
 size_t sys_bytes(const DGrid &g, const Part &P, int pc, bool mixed) {
     const size_t n3 = (size_t)g.n3;
     const size_t sv = mixed ? sizeof(float) : sizeof(double);
     const bool line = pc == PC_LINE;
     size_t b = 0;
-    b += (size_t)(mixed ? 7 : 6) * align256(n3 * sv);
-    if (line) b += 2 * align256(n3 * sv) + align256(4 * n3 * sv);
+    b += (size_t)6 * align256(n3 * sv);
+    if (mixed) b += align256(n3 * sizeof(double));  // y
+    if (line) b += 2 * align256(n3 * sv) + align256((size_t)line_scratch_elems(g) * sv);
     if (mixed) b += align256(8 * n3 * sizeof(float)) + 2 * align256(n3 * sizeof(double));
     b += align256(sizeof(double) * kMaxPartials * (size_t)P.nblk);
     b += align256(sizeof(SolverState));
@@ -346,12 +349,17 @@ void sys_carve(Sys<T> &S, const SysDesc &d, const Part &P, char *base) {
     S.s = (T *)take(n3 * sizeof(T));
     S.t = (T *)take(n3 * sizeof(T));
     S.v = (T *)take(n3 * sizeof(T));
-    if (mixed) S.x = (T *)take(n3 * sizeof(T));
-    else S.x = (T *)d.x;
+    if (mixed) {
+        S.x = nullptr;
+        S.y = (double *)take(n3 * sizeof(double));
+    } else {
+        S.x = (T *)d.x;
+        S.y = nullptr;
+    }
     if (d.pc == PC_LINE) {
         S.phat = (T *)take(n3 * sizeof(T));
         S.shat = (T *)take(n3 * sizeof(T));
-        S.line_scratch = (T *)take(4 * n3 * sizeof(T));
+        S.line_scratch = (T *)take((size_t)line_scratch_elems(d.g) * sizeof(T));
     }
     if (mixed) {
         S.a = (const T *)take(8 * n3 * sizeof(float));
