@@ -309,13 +309,14 @@ def main(argv=None):
     value = args.steps / total
     fp64_value = len(fp64_times) / sum(fp64_times) if fp64_times else None
 
-    # dominant kernel: the FP32 stencil kernel (V = A*Phat and T = A*Shat with
-    # their dot products), 7 A32 planes + operand + dot partner + result = 40 B/cell
+    # dominant kernel: the fused first stencil pass k_p_matvec<float> (P update,
+    # V = A*(P/A0), <Rtilde,V>): 7 coefficient planes + A0 + R, P, V, Rtilde
+    # read + P, V written = 8*4 + 6*4 = 56 B/cell (HIP events in this run)
     kern_avg_s = prof["total_ms"] * 1e-3 / max(1, prof["launches"])
     kern_bytes = prof["alg_bytes"] / max(1, prof["launches"])
     achieved = kern_bytes / kern_avg_s if kern_avg_s > 0 else 0.0
     traffic = None
-    tf = ROOT / "profiles" / "stencil_fp32_256cubed_traffic.json"
+    tf = ROOT / "profiles" / "p_matvec_fp32_256cubed_traffic.json"
     if tf.exists():
         try:
             traffic = json.loads(tf.read_text()).get("bytes_per_launch")
@@ -365,7 +366,7 @@ def main(argv=None):
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_matvec_dots<float> (FP32 stencil + dots)",
+            "kernel": "k_p_matvec<float> (P update + FP32 stencil A*(P/A0) + <Rtilde,V>)",
             "achieved": achieved / 1e9,
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",

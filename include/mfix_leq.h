@@ -105,8 +105,10 @@ typedef struct mfix_leq_opts {
     int write_back_scaled;       /* host entry: copy scaled A_M/B_M back (1 = reference behaviour) */
     const double *noise;         /* optional U[0,1) array (DIMENSION_3) for Rtilde (leq_bicgs.f:298-305); device ptr for _dev calls, host ptr for host calls; NULL = internal generator */
     uint64_t noise_seed;         /* seed of the internal counter-based U[0,1) generator */
-    double inner_tol;            /* mixed precision: relative tolerance of each FP32 inner solve (0 = default 1e-5) */
-    int max_refine;              /* mixed precision: max FP64 refinement cycles (0 = default 40) */
+    double inner_tol;            /* mixed precision: each FP32 cycle reduces the recursive residual by this
+                                    factor before an FP64 reliable update (0 = default 0.1) */
+    int max_refine;              /* mixed precision: end the solve after this many reliable updates
+                                    (IER -1/-2 by the true residual); 0 = no limit */
 } mfix_leq_opts;
 
 /* Device-resident FP64 LEQ_BICGS: var/a_m/b_m are device pointers (storage

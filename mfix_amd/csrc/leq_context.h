@@ -61,6 +61,8 @@ struct MapSet {
     int nlpass = 0;
     std::vector<int> line_seq;
     double cells = 0.0;           // sum of N3 over ids (timing bytes)
+    double k1_bytes = 0.0;        // algorithmic bytes of one first-stencil launch over all ids
+    int n_line = 0, n_fused = 0;  // LINE systems / DIAG+NONE systems among ids
     DevBuf dev;
     int *d_ids = nullptr;
     int2 *d_smap = nullptr, *d_fmap = nullptr, *d_lmap[kMaxRPasses] = {};
@@ -132,6 +134,7 @@ struct SysDesc {
     int itmax;
     int minimize, icheck;
     double delta;                 // mixed: reliable-update trigger
+    int max_updates;              // mixed: reliable updates allowed (0 = no limit)
     double *x, *a, *b;            // VAR, A_M, B_M (FP64, device)
     const double *noise;
     uint64_t seed;

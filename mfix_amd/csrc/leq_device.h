@@ -120,10 +120,21 @@ __device__ __forceinline__ Nb neighbours_fast(const DGrid &g, int64_t c) {
     return n;
 }
 
+// Pointers read from a system table are generic; these casts tell the
+// compiler they address global memory, so loads and stores use the global
+// (not flat) instructions, which do not make LDS waits also wait for them.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T *;
+template <typename T>
+__device__ __forceinline__ gptr<T> to_global(T *p) {
+    return (gptr<T>)p;
+}
+
+
 // Streaming loads of the coefficient planes (read once per sweep): the
 // non-temporal hint keeps them from displacing the reused vector in L2.
-template <typename T>
-__device__ __forceinline__ T ld_stream(const T *p) {
+template <typename P>
+__device__ __forceinline__ auto ld_stream(P p) {
     return __builtin_nontemporal_load(p);
 }
 
@@ -131,10 +142,10 @@ __device__ __forceinline__ T ld_stream(const T *p) {
 // preconditioner applied on the fly, LEQ_MSOLVE1 leqsol_mod.f:729-744, so
 // Phat and Shat never exist as arrays).  Both forms round exactly like the
 // reference's separate division and matvec.
-template <typename T, bool DIV>
+template <typename T, bool DIV, typename PT = const T *>
 struct Operand {
-    const T *__restrict__ x;
-    const T *__restrict__ a0;
+    PT x;
+    PT a0;
     __device__ __forceinline__ T operator()(int64_t q) const {
         if (DIV) return (T)((double)x[q] / (double)a0[q]);
         return x[q];
@@ -142,9 +153,8 @@ struct Operand {
 };
 
 // y(c) of LEQ_MATVEC in the reference's summation order.
-template <typename T, typename X>
-__device__ __forceinline__ T stencil_row(const DGrid &g, const T *__restrict__ a, const X &x, int64_t c,
-                                         const Nb &n) {
+template <typename T, typename X, typename PA>
+__device__ __forceinline__ T stencil_row(const DGrid &g, PA a, const X &x, int64_t c, const Nb &n) {
     const int64_t n3 = g.n3;
     T v = ld_stream(&a[1 * n3 + c]) * x(n.jm) + ld_stream(&a[2 * n3 + c]) * x(n.im);
     v = v + ld_stream(&a[3 * n3 + c]) * x(c);
@@ -162,9 +172,8 @@ __device__ __forceinline__ T stencil_row(const DGrid &g, const T *__restrict__ a
 // y = sum_l a_l (x_nb - x_c) + e x_c is algebraically LEQ_MATVEC but avoids
 // the cancellation between A0 x_c and the neighbour terms that costs a
 // diffusion operator's smooth modes most of FP32's digits.
-template <typename X>
-__device__ __forceinline__ float stencil_row_diff(const DGrid &g, const float *__restrict__ a, const X &x,
-                                                  int64_t c, const Nb &n) {
+template <typename X, typename PA>
+__device__ __forceinline__ float stencil_row_diff(const DGrid &g, PA a, const X &x, int64_t c, const Nb &n) {
     const int64_t n3 = g.n3;
     const float xc = x(c);
     double v = (double)(ld_stream(&a[1 * n3 + c]) * (x(n.jm) - xc)) +
@@ -254,8 +263,8 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double *lds) {
 // block-index order.  The kernel boundary makes the partials visible across
 // the eight XCDs' L2s once per launch, where a per-block release fence
 // (last-block-done pattern) would write back each XCD's L2 for every block.
-template <int BS, int NV>
-__device__ __forceinline__ void block_partial(double (&v)[NV], double *partials, int64_t nblk, int64_t blk,
+template <int BS, int NV, typename PD>
+__device__ __forceinline__ void block_partial(double (&v)[NV], PD partials, int64_t nblk, int64_t blk,
                                               double *lds) {
     block_sum<BS, NV>(v, lds);
     if (threadIdx.x == 0) {
@@ -275,20 +284,21 @@ __device__ __forceinline__ void block_partial(double (&v)[NV], double *partials,
 // reset there for the next launch.
 constexpr int kFinishBlocks = 64;
 
-__device__ __forceinline__ double load_agent(const double *p) {
+template <typename P>
+__device__ __forceinline__ double load_agent(P p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-template <int NV>
-__device__ __forceinline__ bool finish_sum(const double *partials, int64_t nblk, int fb, int nfb, double *l2,
-                                           unsigned *counter, double *lds, double (&tot)[NV]) {
+template <int NV, typename PP, typename PL, typename PC>
+__device__ __forceinline__ bool finish_sum(PP partials, int64_t nblk, int fb, int nfb, PL l2, PC counter,
+                                           double *lds, double (&tot)[NV]) {
     __shared__ bool last;
     const int64_t per = (nblk + nfb - 1) / nfb;
     const int64_t lo = (int64_t)fb * per;
     const int64_t hi = lo + per < nblk ? lo + per : nblk;
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
-        const double *p = partials + q * nblk;
+        const auto p = partials + q * nblk;
         double s = 0.0;
         for (int64_t b = lo + threadIdx.x; b < hi; b += kFinishBlock) s += p[b];
         tot[q] = s;
@@ -299,7 +309,7 @@ __device__ __forceinline__ bool finish_sum(const double *partials, int64_t nblk,
 #pragma unroll
         for (int q = 0; q < NV; ++q) l2[q * nfb + fb] = tot[q];
         __threadfence();
-        const unsigned prev = atomicAdd(counter, 1u);
+        const unsigned prev = atomicAdd((unsigned *)counter, 1u);
         last = prev == (unsigned)nfb - 1;
     }
     __syncthreads();

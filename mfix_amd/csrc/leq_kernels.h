@@ -31,8 +31,8 @@ struct SolverState {
     double tol, rtv, txs, txt;
     double true_rnorm;      // mixed: last FP64 ||b - A x||
     double upd_rnorm;       // mixed: ||r|| at the last reliable update
-    double rnorm_max;       // mixed: max recursive ||r|| since the last update
     double delta;           // mixed: reliable-update trigger factor
+    int max_updates;        // mixed: reliable updates allowed (0 = no limit)
     int i;                  // current loop index (1-based), leq_bicgs.f:316
     int iter_var;           // the reference's `iter` counter
     int itmax;
@@ -46,10 +46,10 @@ struct SolverState {
     int final_update;       // mixed: the pending update ends the solve
     int y_reset;            // mixed: the FP64 correction accumulator restarts from 0
     int discard_y;          // mixed: a non-finite scalar appeared; the next update drops y
+    int fused;              // DIAG/NONE: the S update and ||S|| come with the second stencil pass
     int x_cur;              // mixed: which FP64 solution buffer is current
-    int breakdown;          // mixed: rho = 0 seen before the final update
     int restart;            // mixed: restart the recurrence (p = r, Rtilde = r) at the next iteration
-    int restarts;           // mixed: restarts taken
+    int restarts;           // mixed: recurrence restarts (one per reliable update that did not end the solve)
     unsigned fin_counter;   // arrivals of the current finish launch (reset by its last block)
     int pad_;
     double l2[4 * kFinishBlocks];  // level-2 sums of the finish launches
@@ -67,16 +67,14 @@ template <> struct DiagPlane<float> { static constexpr int value = 7; };
 
 template <typename T> struct Stencil;
 template <> struct Stencil<double> {
-    template <typename X>
-    __device__ __forceinline__ static double row(const DGrid &g, const double *__restrict__ a, const X &x, int64_t c,
-                                                 const Nb &n) {
+    template <typename X, typename PA>
+    __device__ __forceinline__ static double row(const DGrid &g, PA a, const X &x, int64_t c, const Nb &n) {
         return stencil_row<double>(g, a, x, c, n);
     }
 };
 template <> struct Stencil<float> {
-    template <typename X>
-    __device__ __forceinline__ static float row(const DGrid &g, const float *__restrict__ a, const X &x, int64_t c,
-                                                const Nb &n) {
+    template <typename X, typename PA>
+    __device__ __forceinline__ static float row(const DGrid &g, PA a, const X &x, int64_t c, const Nb &n) {
         return stencil_row_diff(g, a, x, c, n);
     }
 };
@@ -88,7 +86,8 @@ template <> struct Stencil<float> {
          C < _end; C += (BS))
 
 // Final IER classification after ITMAX trips, leq_bicgs.f:486-495.
-__device__ __forceinline__ void finish_itmax(SolverState *st) {
+template <typename PS>
+__device__ __forceinline__ void finish_itmax(PS st) {
     int conv = st->isconverged;
     const double rn = (double)(float)st->rnorm;
     if (!conv) conv = rn <= st->tol * st->rnorm0;
@@ -110,32 +109,32 @@ enum Fin : int {
     FIN_SHADOW,         // rho_0 (:318) and its zero test at i=1 (:320-332)
     FIN_ALPHA,          // alpha = rho/<Rtilde,V> (:357-361)
     FIN_SNORM,          // ||S|| <= EPSILON exit request (:372-374)
-    FIN_OMEGA,          // omega = <T,S>/<T,T>, TxT=0 -> SMALL_NUMBER (:411-429)
+    FIN_OMEGA,          // omega = <T,S>/<T,T>, TxT=0 -> SMALL_NUMBER (:411-429); fused ||S|| exit
     FIN_XR,             // ||R||, convergence, next rho and beta (:318-343, :439-495)
     FIN_XR_MIXED,       // same for the FP32 inner recurrence, reliable-update trigger
-    FIN_UPDATE,         // reliable update: FP64 true residual, restart decision
+    FIN_UPDATE,         // reliable update: FP64 true residual, convergence, restart
     FIN_DOT,            // standalone dot product
 };
 
 // True when the phase kernel that wrote the partials returned early (its
 // partials are stale): the finish kernel then does nothing either.
-template <int F>
-__device__ __forceinline__ bool fin_skips(const SolverState *st) {
+template <int F, typename PS>
+__device__ __forceinline__ bool fin_skips(PS st) {
     if (F == FIN_SETUP || F == FIN_DOT) return false;
     if (st->done) return true;
     if (F == FIN_OMEGA) return st->s_exit != 0;
+    if (F == FIN_SNORM) return st->fused != 0;  // decided in FIN_OMEGA
     if (F == FIN_UPDATE) return !st->need_update;
     return false;
 }
 
-template <int F, int NV>
-__device__ __forceinline__ void fin_apply(SolverState *st, const double (&tot)[NV], double *out) {
-    SolverState &S = *st;
+template <int F, int NV, typename PS>
+__device__ __forceinline__ void fin_apply(PS st, const double (&tot)[NV], double *out) {
+    auto &S = *st;
     if (F == FIN_SETUP) {
         S.rnorm0 = sqrt(tot[0]);
         S.true_rnorm = S.rnorm0;
         S.upd_rnorm = S.rnorm0;
-        S.rnorm_max = S.rnorm0;
         if (S.zero_row) {
             S.ier = -2;
             S.exit_reason = EXIT_ZERO_ROW;
@@ -159,6 +158,14 @@ __device__ __forceinline__ void fin_apply(SolverState *st, const double (&tot)[N
             if (S.snorm <= kTolMin) S.s_exit = 1;
         }
     } else if (F == FIN_OMEGA) {
+        if (NV > 2 && S.fused && !S.minimize) {
+            // ||S|| of the fused pass (leq_bicgs.f:372-376)
+            S.snorm = sqrt(tot[NV > 2 ? 2 : 0]);
+            if (S.snorm <= kTolMin) {
+                S.s_exit = 1;
+                return;
+            }
+        }
         double txt = tot[1];
         S.txs = tot[0];
         if (txt == 0.0) txt = kSmallNumber;
@@ -201,68 +208,58 @@ __device__ __forceinline__ void fin_apply(SolverState *st, const double (&tot)[N
         S.rho_prev = S.rho;
         S.rho = rho_new;
     } else if (F == FIN_XR_MIXED) {
+        // FP32 inner recurrence.  A reliable update (FP64 true residual) is
+        // requested when the recursive residual fell by delta since the last
+        // one, reached the tolerance, the S-norm exit or rho = 0 occurred, or a
+        // non-finite scalar appeared (then the correction y is dropped).
         S.y_reset = 0;
         S.restart = 0;
+        S.iter_var += 1;
+        const bool last = S.i >= S.itmax;
+        if (!last) S.i += 1;
         if (!isfinite(tot[0]) || !isfinite(tot[1]) || !isfinite(S.alpha) || !isfinite(S.omega)) {
-            // an FP32 breakdown (a zero or overflowing denominator) reached
-            // the recurrence: restart it from the last FP64 solution
             S.need_update = 1;
             S.discard_y = 1;
-            S.iter_var += 1;
-            if (S.i >= S.itmax) {
-                S.final_update = 1;
-                S.exit_reason = EXIT_ITMAX;
-            } else {
-                S.i += 1;
-            }
-            return;
-        }
-        if (S.s_exit) {
+        } else if (S.s_exit) {
             S.need_update = 1;
-            S.final_update = 1;
-            S.exit_reason = EXIT_SNORM;
-            return;
+        } else {
+            S.rnorm = sqrt(tot[0]);
+            const double rho_new = tot[1];
+            if (S.rnorm <= S.tol * S.rnorm0 || S.rnorm <= S.delta * S.upd_rnorm || rho_new == 0.0) {
+                S.need_update = 1;
+            } else {
+                S.beta = (rho_new / S.rho) * (S.alpha / S.omega);
+                S.rho_prev = S.rho;
+                S.rho = rho_new;
+            }
         }
-        S.rnorm = sqrt(tot[0]);
-        if (S.rnorm > S.rnorm_max) S.rnorm_max = S.rnorm;
-        S.iter_var += 1;
-        if (S.rnorm <= S.tol * S.rnorm0 || S.rnorm <= S.delta * S.upd_rnorm) S.need_update = 1;
-        if (S.i >= S.itmax) {
+        if (last || (S.need_update && S.max_updates > 0 && S.updates + 1 >= S.max_updates)) {
             S.need_update = 1;
             S.final_update = 1;
             S.exit_reason = EXIT_ITMAX;
-            return;
         }
-        S.i += 1;
-        const double rho_new = tot[1];
-        if (S.need_update) return;  // the update restarts the recurrence
-        if (rho_new == 0.0) {
-            S.need_update = 1;
-            S.final_update = 1;
-            S.breakdown = 1;
-            S.exit_reason = EXIT_RHO;
-            return;
-        }
-        S.beta = (rho_new / S.rho) * (S.alpha / S.omega);
-        S.rho_prev = S.rho;
-        S.rho = rho_new;
     } else if (F == FIN_UPDATE) {
-        const bool discarded = S.discard_y;
+        // x <- x + y and the FP64 true residual were formed by the update
+        // kernel.  Converged on ||b - A x|| <= TOL*Rnorm0; otherwise the FP32
+        // recurrence restarts from the true residual with a fresh shadow
+        // residual (P = R, Rtilde = R, rho = <R,R>).  Continuing the old
+        // recurrence instead lets rho = <Rtilde,R> decay below FP32's rounding
+        // level on hard systems, after which alpha and omega are noise.
         S.discard_y = 0;
+        S.s_exit = 0;
         S.x_cur = 1 - S.x_cur;
         S.y_reset = 1;
         S.need_update = 0;
         S.updates += 1;
-        const double prev_true = S.upd_rnorm;
         S.true_rnorm = sqrt(tot[0]);
+        S.upd_rnorm = S.true_rnorm;
+        S.rnorm = S.true_rnorm;
         if (!isfinite(S.true_rnorm)) {
             S.ier = -2;
             S.exit_reason = EXIT_RHO;
             S.done = 1;
             return;
         }
-        S.upd_rnorm = S.true_rnorm;
-        S.rnorm_max = S.true_rnorm;
         if (S.true_rnorm <= S.tol * S.rnorm0) {
             S.isconverged = 1;
             S.ier = 0;
@@ -271,33 +268,18 @@ __device__ __forceinline__ void fin_apply(SolverState *st, const double (&tot)[N
             return;
         }
         if (S.final_update) {
+            // ITMAX reached: classify like leq_bicgs.f:486-495 on the true residual
             S.ier = S.true_rnorm < kRatioTol * S.rnorm0 ? -1 : -2;
-            if (S.breakdown) {
-                S.ier = -2;
-                S.exit_reason = EXIT_RHO;
-            }
             S.done = 1;
             return;
         }
-        // Continue the recurrence from the replaced residual while the
-        // updates make progress; restart it (p = r, Rtilde = r, rho = <r,r>)
-        // when the true residual grew since the last update or the FP32
-        // recurrence drifted more than 10x away from it.
-        const bool drift = discarded || S.true_rnorm > prev_true || S.true_rnorm > 10.0 * S.rnorm;
-        const double rho_cont = tot[1];
-        if (drift || rho_cont == 0.0) {
-            S.restart = 1;
-            S.restarts += 1;
-            S.rho = tot[2];
-            if (S.rho == 0.0) {
-                S.ier = -2;
-                S.exit_reason = EXIT_RHO;
-                S.done = 1;
-            }
-        } else {
-            S.beta = (rho_cont / S.rho) * (S.alpha / S.omega);
-            S.rho_prev = S.rho;
-            S.rho = rho_cont;
+        S.restart = 1;
+        S.restarts += 1;
+        S.rho = tot[1];
+        if (S.rho == 0.0) {
+            S.ier = -2;
+            S.exit_reason = EXIT_RHO;
+            S.done = 1;
         }
     } else if (F == FIN_DOT) {
         *out = tot[0];

@@ -32,10 +32,11 @@ struct Sys {
     Part P;                // cell split: P.nblk logical blocks of P.chunk cells
     int nfin;              // blocks of this system in finish launches
     int pc;                // PC_NONE / PC_DIAG / PC_LINE
-    int itmax, minimize, icheck, mixed;
+    int itmax, minimize, icheck, mixed, max_updates;
     double tol, delta;
     const T *a;            // reference layout (FP64) or difference layout (FP32)
     T *x, *r, *rt, *p, *s, *t, *v, *phat, *shat;  // x: VAR (FP64 solve only)
+    T *p2, *v2;            // DIAG/NONE: P and V of odd iterations (P and V hold even ones)
     double *y;             // mixed: FP64 correction accumulator since the last reliable update
     T *line_scratch;       // 4 planes of DGTSV scratch (LINE)
     SolverState *st;
@@ -70,6 +71,8 @@ __global__ void k_init_states(const Sys<T> *__restrict__ sys, const int *__restr
     s.icheck = S.icheck > 0 ? S.icheck : 1;
     s.mixed = S.mixed;
     s.delta = S.delta;
+    s.max_updates = S.max_updates;
+    s.fused = S.pc != PC_LINE;
     s.true_rnorm = -1.0;
     *S.st = s;
 }
@@ -81,13 +84,13 @@ template <int BS, typename T>
 __global__ void __launch_bounds__(BS) k_setup_residual(const Sys<T> *__restrict__ sys, const int2 *__restrict__ map) {
     __shared__ double lds[(BS / 64) * 1];
     SYS_BLOCK(T)
-    double *__restrict__ a = S.a64;
-    double *__restrict__ b = S.b64;
-    double *__restrict__ r = S.r64;
+    auto *__restrict__ a = to_global(S.a64);
+    auto *__restrict__ b = to_global(S.b64);
+    auto *__restrict__ r = to_global(S.r64);
     double v[1] = {0.0};
     const int64_t n3 = g.n3;
     int zero = 0;
-    const Operand<double, false> X{S.xa, nullptr};
+    const Operand<double, false, gptr<const double>> X{to_global((const double *)S.xa), nullptr};
     FOR_CELLS(BS, P, blk, c) {
         double amax = 0.0;
 #pragma unroll
@@ -105,8 +108,8 @@ __global__ void __launch_bounds__(BS) k_setup_residual(const Sys<T> *__restrict_
         r[c] = rr;
         if (g.dot_all || n.interior) v[0] += rr * rr;
     }
-    if (zero) atomicOr(&S.st->zero_row, 1);
-    block_partial<BS, 1>(v, S.partials, P.nblk, blk, lds);
+    if (zero) atomicOr((int *)&to_global(S.st)->zero_row, 1);
+    block_partial<BS, 1>(v, to_global(S.partials), P.nblk, blk, lds);
 }
 
 // Rtilde = R + (2U-1)*1e-6*Rnorm0 (leq_bicgs.f:298-305) and rho_0 = <Rtilde,R>
@@ -116,25 +119,25 @@ template <int BS, typename T>
 __global__ void __launch_bounds__(BS) k_shadow(const Sys<T> *__restrict__ sys, const int2 *__restrict__ map) {
     __shared__ double lds[(BS / 64) * 1];
     SYS_BLOCK(T)
-    if (S.st->done) return;
+    if (to_global(S.st)->done) return;
     double v[1] = {0.0};
-    const double rn0 = S.st->rnorm0;
-    const double *__restrict__ r64 = S.r64;
-    T *__restrict__ rt = S.rt;
+    const double rn0 = to_global(S.st)->rnorm0;
+    auto *__restrict__ r64 = to_global(S.r64);
+    auto *__restrict__ rt = to_global(S.rt);
     const bool mixed = S.mixed;
     FOR_CELLS(BS, P, blk, c) {
-        const double u = S.noise ? S.noise[c] : urand(S.seed, 100, (uint64_t)c);
+        const double u = to_global(S.noise) ? to_global(S.noise)[c] : urand(S.seed, 100, (uint64_t)c);
         const double rc = r64[c];
         const T t = (T)(rc + (2.0 * u - 1.0) * 1.0e-6 * rn0);
         rt[c] = t;
         const T rT = (T)rc;
         if (mixed) {
-            S.r[c] = rT;
-            S.y[c] = 0.0;
+            to_global(S.r)[c] = rT;
+            to_global(S.y)[c] = 0.0;
         }
         if (g.dot_all || is_interior(g, c)) v[0] += (double)t * (double)rT;
     }
-    block_partial<BS, 1>(v, S.partials, P.nblk, blk, lds);
+    block_partial<BS, 1>(v, to_global(S.partials), P.nblk, blk, lds);
 }
 
 // P = R (i=1, or after a restart) or P = R + beta*(P - omega*V)
@@ -144,18 +147,19 @@ __global__ void __launch_bounds__(BS) k_shadow(const Sys<T> *__restrict__ sys, c
 template <int BS, typename T>
 __global__ void __launch_bounds__(BS) k_p_update(const Sys<T> *__restrict__ sys, const int2 *__restrict__ map) {
     SYS_BLOCK(T)
-    const SolverState *st = S.st;
+    if (S.pc != PC_LINE) return;  // DIAG/NONE: fused into k_p_matvec
+    auto *__restrict__ st = to_global(S.st);
     if (st->done) return;
     const bool restart = st->restart;
     const bool first = st->i == 1 || restart;
     const double beta = st->beta, omega = st->omega;
-    const T *__restrict__ r = S.r;
-    T *__restrict__ p = S.p;
-    const T *__restrict__ v = S.v;
-    T *__restrict__ ph = S.pc == PC_LINE ? S.phat : nullptr;
+    auto *__restrict__ r = to_global(S.r);
+    auto *__restrict__ p = to_global(S.p);
+    auto *__restrict__ v = to_global(S.v);
+    const auto ph = S.pc == PC_LINE ? to_global(S.phat) : nullptr;
     FOR_CELLS(BS, P, blk, c) {
         const T rc = r[c];
-        if (restart) S.rt[c] = rc;
+        if (restart) to_global(S.rt)[c] = rc;
         double pv;
         if (first) pv = (double)rc;
         else pv = (double)rc + beta * ((double)p[c] - omega * (double)v[c]);
@@ -164,24 +168,24 @@ __global__ void __launch_bounds__(BS) k_p_update(const Sys<T> *__restrict__ sys,
     }
 }
 
-// MODE 0: V = A*Phat (leq_bicgs.f:355) and <Rtilde,V> (:357).
-// MODE 1: T = A*Shat (:409), <T,S>, <T,T> (:411-421).
-// The operand is P/A0 or S/A0 for DIAG, the preconditioned array for LINE,
-// and P or S for NONE.
+// LINE systems: MODE 0: V = A*Phat (leq_bicgs.f:355) and <Rtilde,V> (:357);
+// MODE 1: T = A*Shat (:409), <T,S>, <T,T> (:411-421), with Phat/Shat the
+// line-preconditioned arrays.
 template <int BS, typename T, int MODE>
 __global__ void __launch_bounds__(BS) k_matvec_dots(const Sys<T> *__restrict__ sys, const int2 *__restrict__ map) {
     __shared__ double lds[(BS / 64) * 2];
     SYS_BLOCK(T)
-    const SolverState *st = S.st;
+    if (S.pc != PC_LINE) return;  // DIAG/NONE: k_p_matvec / k_s_matvec
+    auto *__restrict__ st = to_global(S.st);
     if (st->done) return;
     if (MODE == 1 && st->s_exit) return;
     double v[2] = {0.0, 0.0};
     const bool all = MODE == 1 ? (g.dot_all && !st->minimize) : g.dot_all;
-    const T *__restrict__ a = S.a;
-    const T *__restrict__ a0 = a + (int64_t)DiagPlane<T>::value * g.n3;
-    T *__restrict__ y = MODE == 0 ? S.v : S.t;
-    const T *__restrict__ w = MODE == 0 ? S.rt : S.s;
-    const T *__restrict__ xin = MODE == 0 ? (S.pc == PC_LINE ? S.phat : S.p) : (S.pc == PC_LINE ? S.shat : S.s);
+    auto *__restrict__ a = to_global(S.a);
+    const auto a0 = a + (int64_t)DiagPlane<T>::value * g.n3;
+    auto *__restrict__ y = MODE == 0 ? to_global(S.v) : to_global(S.t);
+    auto *__restrict__ w = MODE == 0 ? to_global(S.rt) : to_global(S.s);
+    const auto xin = MODE == 0 ? (S.pc == PC_LINE ? to_global(S.phat) : to_global(S.p)) : (S.pc == PC_LINE ? to_global(S.shat) : to_global(S.s));
     auto body = [&](auto X) {
         FOR_CELLS(BS, P, blk, c) {
             const Nb n = neighbours_fast(g, c);
@@ -194,14 +198,140 @@ __global__ void __launch_bounds__(BS) k_matvec_dots(const Sys<T> *__restrict__ s
             }
         }
     };
-    if (S.pc == PC_DIAG) body(Operand<T, true>{xin, a0});
-    else body(Operand<T, false>{xin, nullptr});
+    if (S.pc == PC_DIAG) body(Operand<T, true, gptr<const T>>{xin, a0});
+    else body(Operand<T, false, gptr<const T>>{xin, nullptr});
     if (MODE == 0) {
         double v1[1] = {v[0]};
-        block_partial<BS, 1>(v1, S.partials, P.nblk, blk, lds);
+        block_partial<BS, 1>(v1, to_global(S.partials), P.nblk, blk, lds);
     } else {
-        block_partial<BS, 2>(v, S.partials, P.nblk, blk, lds);
+        block_partial<BS, 2>(v, to_global(S.partials), P.nblk, blk, lds);
     }
+}
+
+// DIAG/NONE systems.  P and V alternate between two buffers by iteration
+// parity, so a block can recompute its neighbours' P from the previous P
+// and V while other blocks write the new ones.
+template <typename PT>
+struct PVBuf {
+    PT cur, prev;
+};
+template <typename T>
+__device__ __forceinline__ PVBuf<gptr<T>> p_bufs(const Sys<T> &S, int i) {
+    return (i & 1) ? PVBuf<gptr<T>>{to_global(S.p2), to_global(S.p)} : PVBuf<gptr<T>>{to_global(S.p), to_global(S.p2)};
+}
+template <typename T>
+__device__ __forceinline__ PVBuf<gptr<T>> v_bufs(const Sys<T> &S, int i) {
+    return (i & 1) ? PVBuf<gptr<T>>{to_global(S.v2), to_global(S.v)} : PVBuf<gptr<T>>{to_global(S.v), to_global(S.v2)};
+}
+
+// Operand of the first stencil: P_i = R (first/restart) or R + beta*(P - omega*V)
+// (leq_bicgs.f:334-343) evaluated where it is needed, divided by A0 for DIAG
+// (LEQ_MSOLVE1): the same operations as the separate passes, so the same bits.
+template <typename T, bool DIV>
+struct POperand {
+    gptr<const T> r, p, v, a0;
+    double beta, omega;
+    bool first;
+    __device__ __forceinline__ T pval(int64_t q) const {
+        if (first) return r[q];
+        return (T)((double)r[q] + beta * ((double)p[q] - omega * (double)v[q]));
+    }
+    __device__ __forceinline__ T operator()(int64_t q) const {
+        const T pv = pval(q);
+        if (DIV) return (T)((double)pv / (double)a0[q]);
+        return pv;
+    }
+};
+
+// Operand of the second stencil: S = R - alpha*V (leq_bicgs.f:365-367), /A0 for DIAG.
+template <typename T, bool DIV>
+struct SOperand {
+    gptr<const T> r, v, a0;
+    double alpha;
+    __device__ __forceinline__ T sval(int64_t q) const { return (T)((double)r[q] - alpha * (double)v[q]); }
+    __device__ __forceinline__ T operator()(int64_t q) const {
+        const T sv = sval(q);
+        if (DIV) return (T)((double)sv / (double)a0[q]);
+        return sv;
+    }
+};
+
+// P update, V = A*Phat and <Rtilde,V> in one pass (DIAG/NONE systems).
+template <int BS, typename T>
+__global__ void __launch_bounds__(BS) k_p_matvec(const Sys<T> *__restrict__ sys, const int2 *__restrict__ map) {
+    __shared__ double lds[(BS / 64) * 1];
+    SYS_BLOCK(T)
+    if (S.pc == PC_LINE) return;
+    auto *__restrict__ st = to_global(S.st);
+    if (st->done) return;
+    const int it = st->i;
+    const bool restart = st->restart;
+    const PVBuf<gptr<T>> pb = p_bufs(S, it), vb = v_bufs(S, it);
+    auto *__restrict__ a = to_global(S.a);
+    const auto a0 = a + (int64_t)DiagPlane<T>::value * g.n3;
+    auto *__restrict__ r = to_global(S.r);
+    auto *__restrict__ rt = to_global(S.rt);
+    double v[1] = {0.0};
+    auto body = [&](auto X) {
+        FOR_CELLS(BS, P, blk, c) {
+            const Nb n = neighbours_fast(g, c);
+            const T yy = Stencil<T>::row(g, a, X, c, n);
+            pb.cur[c] = X.pval(c);
+            vb.cur[c] = yy;
+            T wv;
+            if (restart) {
+                wv = r[c];
+                rt[c] = wv;
+            } else {
+                wv = rt[c];
+            }
+            if (g.dot_all || n.interior) v[0] += (double)wv * (double)yy;
+        }
+    };
+    const bool first = it == 1 || restart;
+    if (S.pc == PC_DIAG)
+        body(POperand<T, true>{r, pb.prev, vb.prev, a0, st->beta, st->omega, first});
+    else
+        body(POperand<T, false>{r, pb.prev, vb.prev, a0, st->beta, st->omega, first});
+    block_partial<BS, 1>(v, to_global(S.partials), P.nblk, blk, lds);
+}
+
+// S update, T = A*Shat, <T,S>, <T,T> and ||S||^2 in one pass (DIAG/NONE).
+// The S-norm exit (leq_bicgs.f:372-376) is decided with the other scalars;
+// T is then unused, so the iterates are those of the separate passes.
+template <int BS, typename T>
+__global__ void __launch_bounds__(BS) k_s_matvec(const Sys<T> *__restrict__ sys, const int2 *__restrict__ map) {
+    __shared__ double lds[(BS / 64) * 3];
+    SYS_BLOCK(T)
+    if (S.pc == PC_LINE) return;
+    auto *__restrict__ st = to_global(S.st);
+    if (st->done) return;
+    const PVBuf<gptr<T>> vb = v_bufs(S, st->i);
+    auto *__restrict__ a = to_global(S.a);
+    const auto a0 = a + (int64_t)DiagPlane<T>::value * g.n3;
+    auto *__restrict__ r = to_global(S.r);
+    auto *__restrict__ s = to_global(S.s);
+    auto *__restrict__ t = to_global(S.t);
+    const bool all = g.dot_all && !st->minimize;
+    double v[3] = {0.0, 0.0, 0.0};
+    auto body = [&](auto X) {
+        FOR_CELLS(BS, P, blk, c) {
+            const Nb n = neighbours_fast(g, c);
+            const T yy = Stencil<T>::row(g, a, X, c, n);
+            const T sv = X.sval(c);
+            s[c] = sv;
+            t[c] = yy;
+            const double yd = (double)yy, sd = (double)sv;
+            if (all || n.interior) {
+                v[0] += yd * sd;
+                v[1] += yd * yd;
+            }
+            if (g.dot_all || n.interior) v[2] += sd * sd;
+        }
+    };
+    if (S.pc == PC_DIAG) body(SOperand<T, true>{r, vb.cur, a0, st->alpha});
+    else body(SOperand<T, false>{r, vb.cur, a0, st->alpha});
+    block_partial<BS, 3>(v, to_global(S.partials), P.nblk, blk, lds);
 }
 
 // S = R - alpha*V (leq_bicgs.f:365-367) and ||S||^2 (:373); for LINE also
@@ -210,21 +340,22 @@ template <int BS, typename T>
 __global__ void __launch_bounds__(BS) k_s_update(const Sys<T> *__restrict__ sys, const int2 *__restrict__ map) {
     __shared__ double lds[(BS / 64) * 1];
     SYS_BLOCK(T)
-    const SolverState *st = S.st;
+    if (S.pc != PC_LINE) return;  // DIAG/NONE: fused into k_s_matvec
+    auto *__restrict__ st = to_global(S.st);
     if (st->done) return;
     double acc[1] = {0.0};
     const double alpha = st->alpha;
-    const T *__restrict__ r = S.r;
-    const T *__restrict__ v = S.v;
-    T *__restrict__ s = S.s;
-    T *__restrict__ sh = S.pc == PC_LINE ? S.shat : nullptr;
+    auto *__restrict__ r = to_global(S.r);
+    auto *__restrict__ v = to_global(S.v);
+    auto *__restrict__ s = to_global(S.s);
+    const auto sh = S.pc == PC_LINE ? to_global(S.shat) : nullptr;
     FOR_CELLS(BS, P, blk, c) {
         const T sv = (T)((double)r[c] - alpha * (double)v[c]);
         s[c] = sv;
         if (sh) sh[c] = sv;
         if (g.dot_all || is_interior(g, c)) acc[0] += (double)sv * (double)sv;
     }
-    block_partial<BS, 1>(acc, S.partials, P.nblk, blk, lds);
+    block_partial<BS, 1>(acc, to_global(S.partials), P.nblk, blk, lds);
 }
 
 // Var += alpha*Phat + omega*Shat; R = S - omega*T (leq_bicgs.f:432-436);
@@ -237,23 +368,23 @@ template <int BS, typename T>
 __global__ void __launch_bounds__(BS) k_xr_update(const Sys<T> *__restrict__ sys, const int2 *__restrict__ map) {
     __shared__ double lds[(BS / 64) * 2];
     SYS_BLOCK(T)
-    const SolverState *st = S.st;
+    auto *__restrict__ st = to_global(S.st);
     if (st->done) return;
     double acc[2] = {0.0, 0.0};
     const double alpha = st->alpha, omega = st->omega;
     const bool sexit = st->s_exit;
     const bool yr = st->y_reset;
     const bool div = S.pc == PC_DIAG;
-    const T *__restrict__ ph = S.pc == PC_LINE ? S.phat : S.p;
-    const T *__restrict__ sh = S.pc == PC_LINE ? S.shat : S.s;
-    const T *__restrict__ a0 = S.a + (int64_t)DiagPlane<T>::value * g.n3;
-    T *__restrict__ x = S.x;
-    double *__restrict__ y = S.y;
+    const auto ph = S.pc == PC_LINE ? to_global(S.phat) : p_bufs(S, st->i).cur;
+    const auto sh = S.pc == PC_LINE ? to_global(S.shat) : to_global(S.s);
+    auto *__restrict__ a0 = to_global(S.a) + (int64_t)DiagPlane<T>::value * g.n3;
+    auto *__restrict__ x = to_global(S.x);
+    auto *__restrict__ y = to_global(S.y);
     const bool mixed = S.mixed;
-    const T *__restrict__ s = S.s;
-    const T *__restrict__ t = S.t;
-    T *__restrict__ r = S.r;
-    const T *__restrict__ rt = S.rt;
+    auto *__restrict__ s = to_global(S.s);
+    auto *__restrict__ t = to_global(S.t);
+    auto *__restrict__ r = to_global(S.r);
+    auto *__restrict__ rt = to_global(S.rt);
     FOR_CELLS(BS, P, blk, c) {
         const double x0 = mixed ? (yr ? 0.0 : y[c]) : (double)x[c];
         double phv, d0 = 1.0;
@@ -278,33 +409,33 @@ __global__ void __launch_bounds__(BS) k_xr_update(const Sys<T> *__restrict__ sys
             }
         }
     }
-    block_partial<BS, 2>(acc, S.partials, P.nblk, blk, lds);
+    block_partial<BS, 2>(acc, to_global(S.partials), P.nblk, blk, lds);
 }
 
 // Reliable update (mixed path): xo = xi + y, r64 = b - A64*(xi + y) with the
 // neighbours' xi + y formed on the fly (xi is read-only here, xo is the other
-// buffer), r = (float)r64; partial ||r64||^2, <Rtilde, r> and <r, r>.  After
-// a breakdown (discard_y) y is dropped: xo = xi.
+// buffer), r = (float)r64; partial ||r64||^2 and <r, r> (rho of the restarted
+// recurrence, whose Rtilde is r).  After a breakdown (discard_y) y is
+// dropped: xo = xi.
 template <int BS>
 __global__ void __launch_bounds__(BS) k_reliable_update(const Sys<float> *__restrict__ sys,
                                                         const int2 *__restrict__ map) {
-    __shared__ double lds[(BS / 64) * 3];
+    __shared__ double lds[(BS / 64) * 2];
     SYS_BLOCK(float)
-    const SolverState *st = S.st;
+    auto *__restrict__ st = to_global(S.st);
     if (st->done || !st->need_update) return;
     const int cur = st->x_cur;
-    const double *__restrict__ xi = cur ? S.xb : S.xa;
-    double *__restrict__ xo = cur ? S.xa : S.xb;
-    const double *__restrict__ a = S.a64;
-    const double *__restrict__ b = S.b64;
-    float *__restrict__ r = S.r;
-    const float *__restrict__ rt = S.rt;
-    double acc[3] = {0.0, 0.0, 0.0};
+    const auto xi = cur ? to_global(S.xb) : to_global(S.xa);
+    const auto xo = cur ? to_global(S.xa) : to_global(S.xb);
+    auto *__restrict__ a = to_global(S.a64);
+    auto *__restrict__ b = to_global(S.b64);
+    auto *__restrict__ r = to_global(S.r);
+    double acc[2] = {0.0, 0.0};
     struct XY {
-        const double *__restrict__ xi;
-        const double *__restrict__ y;
+        gptr<const double> xi;
+        gptr<const double> y;
         __device__ __forceinline__ double operator()(int64_t q) const { return y ? xi[q] + y[q] : xi[q]; }
-    } X{xi, st->discard_y ? nullptr : S.y};
+    } X{xi, st->discard_y ? nullptr : to_global(S.y)};
     FOR_CELLS(BS, P, blk, c) {
         const Nb n = neighbours_fast(g, c);
         const double rr = b[c] - stencil_row<double>(g, a, X, c, n);
@@ -313,11 +444,10 @@ __global__ void __launch_bounds__(BS) k_reliable_update(const Sys<float> *__rest
         r[c] = r32;
         if (g.dot_all || n.interior) {
             acc[0] += rr * rr;
-            acc[1] += (double)rt[c] * (double)r32;
-            acc[2] += (double)r32 * (double)r32;
+            acc[1] += (double)r32 * (double)r32;
         }
     }
-    block_partial<BS, 3>(acc, S.partials, P.nblk, blk, lds);
+    block_partial<BS, 2>(acc, to_global(S.partials), P.nblk, blk, lds);
 }
 
 // FP32 difference-layout copy of the (scaled) FP64 matrix: off-diagonals
@@ -327,9 +457,9 @@ __global__ void __launch_bounds__(BS) k_reliable_update(const Sys<float> *__rest
 template <int BS>
 __global__ void __launch_bounds__(BS) k_prepare_a32(const Sys<float> *__restrict__ sys, const int2 *__restrict__ map) {
     SYS_BLOCK(float)
-    if (S.st->done) return;
-    const double *__restrict__ a = S.a64;
-    float *__restrict__ a32 = const_cast<float *>(S.a);
+    if (to_global(S.st)->done) return;
+    auto *__restrict__ a = to_global(S.a64);
+    auto *__restrict__ a32 = to_global(const_cast<float *>(S.a));
     const int64_t n3 = g.n3;
     FOR_CELLS(BS, P, blk, c) {
         double e = a[3 * n3 + c];
@@ -354,45 +484,214 @@ __global__ void __launch_bounds__(kFinishBlock) k_finish_sys(const Sys<T> *__res
     const int2 e = map[blockIdx.x];
     if (e.x < 0) return;
     const Sys<T> &S = sys[e.x];
-    SolverState *st = S.st;
+    auto *__restrict__ st = to_global(S.st);
     if (fin_skips<F>(st)) return;
     double tot[NV];
-    if (finish_sum<NV>(S.partials, S.P.nblk, e.y, S.nfin, st->l2, &st->fin_counter, lds, tot) && threadIdx.x == 0)
+    if (finish_sum<NV>(to_global(S.partials), S.P.nblk, e.y, S.nfin, st->l2, &st->fin_counter, lds, tot) && threadIdx.x == 0)
         fin_apply<F, NV>(st, tot, nullptr);
 }
 
 // Red-black line phase along J for every LINE system of a table (RSRS and
-// the 2-D R sweep): block b solves lines [map[b].y*64, +64) of colour
-// `param` of system map[b].x, numbered densely by color_line.  which = 0
-// preconditions P into Phat, 1 S into Shat (skipped after the S-norm exit).
+// the 2-D R sweep).  Block b takes lines [map[b].y*64, +64) of colour
+// `param` of system map[b].x, numbered densely by color_line; lane l owns
+// line l of the block and runs DGTSV on it (same operations and order as
+// solve_line, so the result is bit-identical).  Lines along J are contiguous,
+// so the block reads rows in tiles of 8: element k of a lane is row
+// (lane & 7) of line (lane >> 3) + 8k, and each load instruction fetches 8
+// lines x 64 contiguous bytes instead of 64 scattered words.  The right-hand
+// side (B minus the I/K neighbour terms, whose offsets are constant along a
+// J-line) is formed as the tile is loaded, and the tile is transposed through
+// LDS to the owning lanes.  The next tile's loads are in flight while the
+// current tile is eliminated.  Finished rows go to scratch in [row][lane]
+// order (coalesced), are read back in groups of 8 rows by the back solve, and
+// the solution is transposed through LDS again for coalesced stores.  which
+// = 0 preconditions P into Phat, 1 S into Shat (skipped after the S-norm exit).
 template <typename T>
 __global__ void __launch_bounds__(64) k_line_color_sys(const Sys<T> *__restrict__ sys, const int2 *__restrict__ map,
                                                        int param, int which) {
+    constexpr int R = 8;
+    __shared__ T tl[2][4][R][64];  // dl, d, du, bb: R rows x 64 lines, double buffered
     const int2 e = map[blockIdx.x];
     if (e.x < 0) return;
     const Sys<T> &S = sys[e.x];
-    const SolverState *st = S.st;
+    auto *__restrict__ st = to_global(S.st);
     if (st->done || (which == 1 && st->s_exit)) return;
     const DGrid &g = S.g;
     const LineGeom L = line_geom(g, AX_J);
-    int u, v;
-    const int64_t lane = (int64_t)e.y * 64 + threadIdx.x;
-    if (!color_line(L.nu, L.nv, g.do_k ? 1 : 0, param, lane, u, v)) return;
-    const int64_t c0 = (int64_t)u * L.su + (int64_t)v * L.sv;
-    // lanes of this system: 64 per block, blocks for the larger colour
-    const int64_t nl = (((int64_t)L.nu * L.nv + 1) / 2 + 63) / 64 * 64;
-    solve_line<T>(g, L, AX_J, c0, S.a, which == 0 ? S.p : S.s, which == 0 ? S.phat : S.shat, S.line_scratch, lane,
-                  nl, 0, nullptr);
+    const int n = L.n;
+    const int64_t n3 = g.n3;
+    const int lane = threadIdx.x;
+    const int do_k = g.do_k;
+    constexpr int pd = DiagPlane<T>::value;
+    // uniform plane bases in the global address space; cells are addressed
+    // with 32-bit offsets (N3 < 2^31)
+    const gptr<const T> a = to_global(S.a);
+    const gptr<const T> pI_m = a + 2 * n3, pI_p = a + 4 * n3, pK_m = a, pK_p = a + 6 * n3;
+    const gptr<const T> pD = a + pd * n3, pJ_m = a + n3, pJ_p = a + 5 * n3;
+    const gptr<const T> bm = to_global((const T *)(which == 0 ? S.p : S.s));
+    const gptr<T> var = to_global(which == 0 ? S.phat : S.shat);
+    const gptr<T> sc = to_global(S.line_scratch + (int64_t)e.y * 4 * n * 64);
+    const uint32_t c1 = (uint32_t)g.c1, c2 = (uint32_t)g.c2;
+    const uint32_t pl = (uint32_t)n * 64;  // finished rows: 4 arrays of n rows x 64 lanes
+    int u = 0, v = 0;
+    const int vld = color_line(L.nu, L.nv, do_k ? 1 : 0, param, (int64_t)e.y * 64 + lane, u, v) ? 1 : 0;
+    const int rsub = lane & 7;
+    // loader view: element k of this lane is row rsub of line (lane >> 3) + 8k,
+    // whose start cell and I/K neighbour offsets (constant along a J-line,
+    // set_increments.f:72-112) are kept per element
+    uint32_t jc[R], jim[R], jip[R], jkm[R], jkp[R];
+    int jok[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const int src = (lane >> 3) + 8 * k;
+        const int iu = __shfl(u, src, 64), kv = __shfl(v, src, 64);
+        jok[k] = __shfl(vld, src, 64);
+        jc[k] = (uint32_t)iu * c1 + (uint32_t)kv * c2;
+        jim[k] = (uint32_t)(nb_minus(iu, g.ni, g.cyc_x) - iu) * c1;
+        jip[k] = (uint32_t)(nb_plus(iu, g.ni, g.cyc_x) - iu) * c1;
+        jkm[k] = do_k ? (uint32_t)(nb_minus(kv, g.nk, g.cyc_z) - kv) * c2 : 0u;
+        jkp[k] = do_k ? (uint32_t)(nb_plus(kv, g.nk, g.cyc_z) - kv) * c2 : 0u;
+    }
+    // Load tile rows [Q0, Q0+R) and stage them in LDS buffer BUF: the
+    // right-hand side in LEQ_IKSWEEP's order (leqsol_mod.f:891-902).
+#define MFIX_LINE_TILE(Q0, BUF)                                                                          \
+    _Pragma("unroll") for (int k = 0; k < R; ++k) {                                                      \
+        const int q = (Q0) + rsub;                                                                       \
+        T bb = (T)0, dd = (T)0, dl = (T)0, du = (T)0;                                                    \
+        if (jok[k] && q < n) {                                                                           \
+            const uint32_t c = jc[k] + (uint32_t)q;                                                      \
+            if (do_k)                                                                                    \
+                bb = bm[c] - pI_m[c] * var[c + jim[k]] - pI_p[c] * var[c + jip[k]] -                     \
+                     pK_m[c] * var[c + jkm[k]] - pK_p[c] * var[c + jkp[k]];                              \
+            else                                                                                         \
+                bb = bm[c] - pI_m[c] * var[c + jim[k]] - pI_p[c] * var[c + jip[k]];                      \
+            dd = pD[c];                                                                                  \
+            dl = q == 0 ? (T)0 : pJ_m[c];                                                                \
+            du = q == n - 1 ? (T)0 : pJ_p[c];                                                            \
+        }                                                                                                \
+        const int j = (lane >> 3) + 8 * k;                                                               \
+        tl[BUF][0][rsub][j] = dl;                                                                        \
+        tl[BUF][1][rsub][j] = dd;                                                                        \
+        tl[BUF][2][rsub][j] = du;                                                                        \
+        tl[BUF][3][rsub][j] = bb;                                                                        \
+    }
+    const int ntiles = (n + R - 1) / R;
+    MFIX_LINE_TILE(0, 0)
+    __syncthreads();
+    T d0 = (T)0, du0 = (T)0, b0 = (T)0;
+    int info = 0;
+    for (int tt = 0; tt < ntiles; ++tt) {
+        const int buf = tt & 1;
+        const int q0 = tt * R;
+        // DGTSV elimination of this lane's line over the tile (DGTSV.f:107-163,
+        // the operations of solve_line in the same order)
+        for (int r = 0; r < R; ++r) {
+            const int q = q0 + r;
+            if (q >= n) break;
+            const T dl1 = tl[buf][0][r][lane];
+            T d1 = tl[buf][1][r][lane], du1 = tl[buf][2][r][lane], b1 = tl[buf][3][r][lane];
+            if (q == 0) {
+                d0 = d1;
+                du0 = du1;
+                b0 = b1;
+                continue;
+            }
+            const int i = q - 1;
+            T fd, fu, fu2, fb;
+            if (fabs((double)d0) >= fabs((double)dl1)) {
+                if (d0 == (T)0 && !info) info = i + 1;
+                const T fact = dl1 / d0;
+                d1 = d1 - fact * du0;
+                b1 = b1 - fact * b0;
+                fd = d0;
+                fu = du0;
+                fu2 = (T)0;
+                fb = b0;
+            } else {
+                const T fact = d0 / dl1;
+                fd = dl1;
+                T temp = d1;
+                d1 = du0 - fact * temp;
+                if (i < n - 2) {
+                    fu2 = du1;
+                    du1 = -fact * fu2;
+                } else {
+                    fu2 = (T)0;
+                }
+                fu = temp;
+                temp = b0;
+                fb = b1;
+                b1 = temp - fact * b1;
+            }
+            const uint32_t o = (uint32_t)i * 64 + lane;
+            sc[o] = fd;
+            sc[pl + o] = fu;
+            sc[2 * pl + o] = fu2;
+            sc[3 * pl + o] = fb;
+            d0 = d1;
+            du0 = du1;
+            b0 = b1;
+        }
+        if (tt + 1 < ntiles) {
+            if (buf) {
+                MFIX_LINE_TILE((tt + 1) * R, 0)
+            } else {
+                MFIX_LINE_TILE((tt + 1) * R, 1)
+            }
+        }
+        __syncthreads();
+    }
+#undef MFIX_LINE_TILE
+    if (!info && d0 == (T)0) info = n;  // DGTSV.f:160-163
+    int jinfo[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) jinfo[k] = __shfl(info, (lane >> 3) + 8 * k, 64);
+    // back solve (DGTSV.f:233-246) in groups of R rows from the top row down;
+    // each group is transposed through LDS for coalesced stores of the line
+    T x1 = b0 / d0, x2 = (T)0;
+    for (int hi = n - 1; hi >= 0; hi -= R) {
+        const int lo = hi - R + 1 > 0 ? hi - R + 1 : 0;
+        for (int q = hi; q >= lo; --q) {
+            T x;
+            const uint32_t o = (uint32_t)q * 64 + lane;
+            if (q == n - 1) {
+                x = x1;
+            } else if (q == n - 2) {
+                x = (sc[3 * pl + o] - sc[pl + o] * x1) / sc[o];
+                x2 = x1;
+                x1 = x;
+            } else {
+                x = (sc[3 * pl + o] - sc[pl + o] * x1 - sc[2 * pl + o] * x2) / sc[o];
+                x2 = x1;
+                x1 = x;
+            }
+            tl[0][0][q - lo][lane] = x;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const int q = lo + rsub;
+            // a line whose elimination hit a zero pivot keeps its values
+            // (leqsol_mod.f:910-914)
+            if (jok[k] && !jinfo[k] && q <= hi) var[jc[k] + (uint32_t)q] = tl[0][0][rsub][(lane >> 3) + 8 * k];
+        }
+        __syncthreads();
+    }
 }
 
-// Number of the listed systems still running.
+// Number of the listed systems still running, and a running flag per
+// listed system.
 template <typename T>
-__global__ void k_count_running(const Sys<T> *__restrict__ sys, const int *__restrict__ ids, int nids, int *out) {
+__global__ void k_count_running(const Sys<T> *__restrict__ sys, const int *__restrict__ ids, int nids, int *out,
+                                int *flags) {
     __shared__ int cnt;
     if (threadIdx.x == 0) cnt = 0;
     __syncthreads();
-    for (int q = threadIdx.x; q < nids; q += blockDim.x)
-        if (!sys[ids[q]].st->done) atomicAdd(&cnt, 1);
+    for (int q = threadIdx.x; q < nids; q += blockDim.x) {
+        const int run = !sys[ids[q]].st->done;
+        flags[q] = run;
+        if (run) atomicAdd(&cnt, 1);
+    }
     __syncthreads();
     if (threadIdx.x == 0) *out = cnt;
 }

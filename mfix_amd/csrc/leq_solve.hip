@@ -303,7 +303,8 @@ double alg_bytes_per_cell(int pc, int sa, int sv, int sx) {
 
 static inline size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
-// FP64: R, Rtilde, P, S, T, V (+ Phat, Shat and the DGTSV scratch for LINE).
+// FP64: R, Rtilde, P, S, T, V (+ Phat, Shat and the DGTSV scratch for LINE,
+// a second P and V for DIAG/NONE, whose fused passes alternate buffers).
 // Mixed: the FP32 r, rt, p, s, t, v (+ LINE arrays), the FP64 correction
 // accumulator y, the 8-plane FP32 matrix, the FP64 residual and second
 // solution buffer.  Both: the
@@ -314,6 +315,7 @@ size_t sys_bytes(const DGrid &g, const Part &P, int pc, bool mixed) {
     const bool line = pc == PC_LINE;
     size_t b = 0;
     b += (size_t)6 * align256(n3 * sv);
+    if (!line) b += (size_t)2 * align256(n3 * sv);  // second P and V buffers
     if (mixed) b += align256(n3 * sizeof(double));  // y
     if (line) b += 2 * align256(n3 * sv) + align256((size_t)line_scratch_elems(g) * sv);
     if (mixed) b += align256(8 * n3 * sizeof(float)) + 2 * align256(n3 * sizeof(double));
@@ -343,6 +345,7 @@ void sys_carve(Sys<T> &S, const SysDesc &d, const Part &P, char *base) {
     S.mixed = mixed ? 1 : 0;
     S.tol = d.tol;
     S.delta = d.delta;
+    S.max_updates = d.max_updates;
     S.r = (T *)take(n3 * sizeof(T));
     S.rt = (T *)take(n3 * sizeof(T));
     S.p = (T *)take(n3 * sizeof(T));
@@ -355,6 +358,13 @@ void sys_carve(Sys<T> &S, const SysDesc &d, const Part &P, char *base) {
     } else {
         S.x = (T *)d.x;
         S.y = nullptr;
+    }
+    if (d.pc != PC_LINE) {
+        S.p2 = (T *)take(n3 * sizeof(T));
+        S.v2 = (T *)take(n3 * sizeof(T));
+    } else {
+        S.p2 = S.p;
+        S.v2 = S.v;
     }
     if (d.pc == PC_LINE) {
         S.phat = (T *)take(n3 * sizeof(T));
@@ -401,11 +411,22 @@ template <typename T>
 int build_maps(const Group<T> &G, const std::vector<int> &ids, MapSet &M, hipStream_t s) {
     std::vector<int2> smap, fmap, lmap[kMaxRPasses];
     std::vector<int> seq;
-    int nlpass = 0;
-    double cells = 0.0;
+    int nlpass = 0, n_line = 0, n_fused = 0;
+    double cells = 0.0, k1 = 0.0;
+    const double sa = sizeof(T), sv = sizeof(T);
     for (int id : ids) {
         const Sys<T> &S = G.sys[id];
         cells += (double)S.g.n3;
+        // first stencil pass: 7 coefficient planes (+ A0 for DIAG), and for
+        // the fused DIAG/NONE pass R, P, V, Rtilde read and P, V written;
+        // for LINE Phat, Rtilde read and V written
+        if (S.pc == PC_LINE) {
+            ++n_line;
+            k1 += (double)S.g.n3 * (7.0 * sa + 3.0 * sv);
+        } else {
+            ++n_fused;
+            k1 += (double)S.g.n3 * ((S.pc == PC_DIAG ? 8.0 : 7.0) * sa + 6.0 * sv);
+        }
         // stream kernels: an 8-aligned range whose slot j holds the logical
         // block that puts one contiguous slab of the system on each XCD
         const int64_t nb = S.P.nblk;
@@ -440,6 +461,9 @@ int build_maps(const Group<T> &G, const std::vector<int> &ids, MapSet &M, hipStr
         same = M.lmap[q].size() == lmap[q].size() &&
                !std::memcmp(M.lmap[q].data(), lmap[q].data(), lmap[q].size() * sizeof(int2));
     M.cells = cells;
+    M.k1_bytes = k1;
+    M.n_line = n_line;
+    M.n_fused = n_fused;
     if (same) return 0;
     M.ids = ids;
     M.smap.swap(smap);
@@ -544,33 +568,37 @@ static void line_sweeps(const Group<T> &G, const MapSet &M, int which, hipStream
     }
 }
 
-// One BiCGSTAB iteration (leq_bicgs.f:316-466) over the table.
+// One BiCGSTAB iteration (leq_bicgs.f:316-466) over the table.  DIAG/NONE
+// systems take the fused passes (P update + first stencil, S update +
+// second stencil); LINE systems the separate passes around the line sweeps.
+// A table with only one kind skips the other kind's launches.
 template <int BS, typename T>
 static void iteration_bs(const Group<T> &G, const MapSet &M, hipStream_t s) {
     const dim3 gs((unsigned)M.smap.size()), blk(BS);
     Prof &pf = ctx().prof[0];
-    // algorithmic bytes of one stencil launch: 7 coefficient planes, the
-    // operand, the dot partner and the result, once each
-    const double mv_bytes = M.cells * (10.0 * sizeof(T));
-    const bool any_line = M.nlpass > 0 || !M.line_seq.empty();
-    hipLaunchKernelGGL((k_p_update<BS, T>), gs, blk, 0, s, G.d_sys, M.d_smap);
-    if (any_line) line_sweeps(G, M, 0, s);
+    const bool line = M.n_line > 0, fused = M.n_fused > 0;
     pf.begin(s);
-    hipLaunchKernelGGL((k_matvec_dots<BS, T, 0>), gs, blk, 0, s, G.d_sys, M.d_smap);
-    pf.end(s, mv_bytes);
+    if (fused) hipLaunchKernelGGL((k_p_matvec<BS, T>), gs, blk, 0, s, G.d_sys, M.d_smap);
+    if (line) {
+        hipLaunchKernelGGL((k_p_update<BS, T>), gs, blk, 0, s, G.d_sys, M.d_smap);
+        line_sweeps(G, M, 0, s);
+        hipLaunchKernelGGL((k_matvec_dots<BS, T, 0>), gs, blk, 0, s, G.d_sys, M.d_smap);
+    }
+    pf.end(s, M.k1_bytes);
     finish<T, FIN_ALPHA, 1>(G, M, s);
-    hipLaunchKernelGGL((k_s_update<BS, T>), gs, blk, 0, s, G.d_sys, M.d_smap);
-    finish<T, FIN_SNORM, 1>(G, M, s);
-    if (any_line) line_sweeps(G, M, 1, s);
-    pf.begin(s);
-    hipLaunchKernelGGL((k_matvec_dots<BS, T, 1>), gs, blk, 0, s, G.d_sys, M.d_smap);
-    pf.end(s, mv_bytes);
-    finish<T, FIN_OMEGA, 2>(G, M, s);
+    if (fused) hipLaunchKernelGGL((k_s_matvec<BS, T>), gs, blk, 0, s, G.d_sys, M.d_smap);
+    if (line) {
+        hipLaunchKernelGGL((k_s_update<BS, T>), gs, blk, 0, s, G.d_sys, M.d_smap);
+        finish<T, FIN_SNORM, 1>(G, M, s);
+        line_sweeps(G, M, 1, s);
+        hipLaunchKernelGGL((k_matvec_dots<BS, T, 1>), gs, blk, 0, s, G.d_sys, M.d_smap);
+    }
+    finish<T, FIN_OMEGA, 3>(G, M, s);
     hipLaunchKernelGGL((k_xr_update<BS, T>), gs, blk, 0, s, G.d_sys, M.d_smap);
     if (G.mixed) {
         finish<T, FIN_XR_MIXED, 2>(G, M, s);
         hipLaunchKernelGGL((k_reliable_update<BS>), gs, blk, 0, s, (const Sys<float> *)G.d_sys, M.d_smap);
-        finish<T, FIN_UPDATE, 3>(G, M, s);
+        finish<T, FIN_UPDATE, 2>(G, M, s);
     } else {
         finish<T, FIN_XR, 2>(G, M, s);
     }
@@ -585,8 +613,31 @@ template void enqueue_iterations<double>(Group<double> &, const MapSet &, int, h
 template void enqueue_iterations<float>(Group<float> &, const MapSet &, int, hipStream_t);
 
 template <typename T>
-static void enqueue_count(const Group<T> &G, const MapSet &M, int *d_out, hipStream_t s) {
-    hipLaunchKernelGGL(k_count_running<T>, dim3(1), dim3(256), 0, s, G.d_sys, M.d_ids, (int)M.ids.size(), d_out);
+static void enqueue_count(const Group<T> &G, const MapSet &M, int *d_out, int *d_flags, hipStream_t s) {
+    hipLaunchKernelGGL(k_count_running<T>, dim3(1), dim3(256), 0, s, G.d_sys, M.d_ids, (int)M.ids.size(), d_out,
+                       d_flags);
+}
+
+// Drop finished systems from a table's block maps once the running systems
+// hold less than 60 % of the mapped cells, so later launches stop
+// dispatching their blocks (a handful of rebuilds per batch).
+template <typename T>
+static int compact(Group<T> &G, const int *d_flags, int running, hipStream_t s) {
+    MapSet &M = G.iter;
+    if (running >= (int)M.ids.size() || running == 0) return 0;
+    std::vector<int> flags(M.ids.size());
+    hipError_t e = hipMemcpyAsync(flags.data(), d_flags, sizeof(int) * flags.size(), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(e, "running flags");
+    std::vector<int> keep;
+    double cells = 0.0;
+    for (size_t q = 0; q < flags.size(); ++q)
+        if (flags[q]) {
+            keep.push_back(M.ids[q]);
+            cells += (double)G.sys[M.ids[q]].g.n3;
+        }
+    if (cells >= 0.6 * M.cells) return 0;
+    return build_maps<T>(G, keep, M, s);
 }
 
 // Chunk of iterations of both tables, captured once and replayed.
@@ -621,8 +672,10 @@ static int capture_chunk(Group<double> *G64, Group<float> *G32, int iters, hipSt
 static int run_tables(Group<double> *G64, Group<float> *G32, int max_iters, hipStream_t s, bool graphs) {
     Context &C = ctx();
     int rc;
-    if ((rc = C.misc.ensure(4096))) return rc;
+    if ((rc = C.misc.ensure(256 + 2 * sizeof(int) * kHostStates))) return rc;
     int *d_cnt = (int *)C.misc.ptr;
+    int *d_flags64 = (int *)((char *)C.misc.ptr + 256);
+    int *d_flags32 = d_flags64 + kHostStates;
     if (G64 && G64->iter.ids.empty()) G64 = nullptr;
     if (G32 && G32->iter.ids.empty()) G32 = nullptr;
     if (!G64 && !G32) return 0;
@@ -634,21 +687,25 @@ static int run_tables(Group<double> *G64, Group<float> *G32, int max_iters, hipS
     int *exec_iters = G32 ? &G32->exec_iters : &G64->exec_iters;
     uint64_t *exec_version = G32 ? &G32->exec_version : &G64->exec_version;
     const void **exec_sys = G32 ? &G32->exec_sys : &G64->exec_sys;
-    const uint64_t version = (G64 ? G64->iter.version * 1000003ull : 0) ^ (G32 ? G32->iter.version : 0) ^
-                             ((G64 ? 1ull : 0) << 62) ^ ((G32 ? 1ull : 0) << 63);
     const void *sys_key = G32 ? (const void *)G32->d_sys : (const void *)G64->d_sys;
     const int kGraphIters = 16;
     int64_t issued = 0;
     int chunk = 4;
     for (;;) {
-        if (G64) enqueue_count<double>(*G64, G64->iter, d_cnt + 0, s);
-        if (G32) enqueue_count<float>(*G32, G32->iter, d_cnt + 1, s);
+        if (G64) enqueue_count<double>(*G64, G64->iter, d_cnt + 0, d_flags64, s);
+        if (G32) enqueue_count<float>(*G32, G32->iter, d_cnt + 1, d_flags32, s);
         hipError_t e = hipMemcpyAsync(C.host_count, d_cnt, 2 * sizeof(int), hipMemcpyDeviceToHost, s);
         if (e == hipSuccess) e = hipStreamSynchronize(s);
         if (e != hipSuccess) return hip_fail(e, "running count");
         const int running = (G64 ? C.host_count[0] : 0) + (G32 ? C.host_count[1] : 0);
         if (running == 0) return 0;
         if (issued > 4 * ((int64_t)max_iters + 8)) return hip_fail(hipErrorUnknown, "solver did not terminate");
+        if (G64 && (rc = compact<double>(*G64, d_flags64, C.host_count[0], s))) return rc;
+        if (G32 && (rc = compact<float>(*G32, d_flags32, C.host_count[1], s))) return rc;
+        if (G64 && G64->iter.ids.empty()) G64 = nullptr;
+        if (G32 && G32->iter.ids.empty()) G32 = nullptr;
+        const uint64_t version = (G64 ? G64->iter.version * 1000003ull : 0) ^ (G32 ? G32->iter.version : 0) ^
+                                 ((G64 ? 1ull : 0) << 62) ^ ((G32 ? 1ull : 0) << 63);
         if (graphs) {
             if (!*exec || *exec_iters != kGraphIters || *exec_version != version || *exec_sys != sys_key) {
                 if (*exec) (void)hipGraphExecDestroy(*exec);
@@ -747,7 +804,7 @@ static int solve_one(Group<T> &G, const SysDesc &d, const Shape &sh, mfix_leq_st
     for (auto &p : C.prof) p.collect();
     if (rc) return rc;
     SolverState *h = C.host_state;
-    if ((rc = read_states<T>(G, G.iter.ids, h, s))) return rc;
+    if ((rc = read_states<T>(G, std::vector<int>{0}, h, s))) return rc;
     if (mixed && h->x_cur == 1) {
         hipError_t e = hipMemcpyAsync(d.x, G.sys[0].xb, sizeof(double) * d.g.n3, hipMemcpyDeviceToDevice, s);
         if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -772,6 +829,7 @@ static SysDesc make_desc(const DGrid &g, int pc, const char *cmethod, double tol
     d.minimize = minimize;
     d.icheck = icheck;
     d.delta = delta;
+    d.max_updates = opts && opts->max_refine > 0 ? opts->max_refine : 0;
     d.x = x;
     d.a = a;
     d.b = b;
