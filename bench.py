@@ -148,6 +148,66 @@ def bench_batch(args):
     return 0
 
 
+def bench_line(args):
+    """Plain FP64 LEQ_BICGS with the LINE preconditioner (RSRS) on the 128^3
+    first-order-upwind momentum system (seed 1000), the reference's own
+    arithmetic; also the mixed-precision solve of the same system."""
+    from mfix_amd import leq, systems, _lib
+    n = 128
+    a, b = systems.gen_system(systems.MOMENTUM, n, n, n, 1000)
+    leq.grid_init(n, n, n)
+    a0, b0 = leq.DeviceArray.from_host(a), leq.DeviceArray.from_host(b)
+    da, db, dx = leq.DeviceArray(a.size), leq.DeviceArray(b.size), leq.DeviceArray(b.size)
+    zeros = np.zeros(b.size)
+
+    def one(mixed):
+        da.copy_from(a0)
+        db.copy_from(b0)
+        dx.upload(zeros)
+        _lib.check(_lib.load().mfix_leq_sync(), "sync")
+        t = time.perf_counter()
+        out = leq.bicgs_dev(dx, da, db, pc="LINE", cmethod="RSRS", tol=1e-10, itmax=500, mixed=mixed)
+        dt = time.perf_counter() - t
+        if out.ier != 0:
+            raise RuntimeError(f"solve failed: {out}")
+        return dt, out
+
+    res = {}
+    for mixed in (False, True):
+        for _ in range(args.warmup):
+            one(mixed)
+        leq.prof_enable(1, True)
+        times, its = [], []
+        for _ in range(args.steps):
+            dt, out = one(mixed)
+            times.append(dt)
+            its.append(out.stats["iterations"])
+        pr = leq.prof_read(1)
+        leq.prof_enable(1, False)
+        res[mixed] = dict(value=len(times) / sum(times), its=float(np.mean(its)), ms=1e3 * sum(times) / len(times),
+                          line_us=1e3 * pr["total_ms"] / max(1, pr["launches"]),
+                          line_GBps=pr["alg_bytes"] / max(1e-12, pr["total_ms"] * 1e-3) / 1e9)
+    f, m = res[False], res[True]
+    n3 = b.size
+    alg = f["its"] * n3 * _bytes_per_cell("LINE", False) / (f["ms"] * 1e-3)
+    line = {
+        "metric": "solves/s (FP64 LEQ_BICGS, LINE/RSRS, 1e-10 relative residual)",
+        "value": f["value"], "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": f["ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"upwind momentum {n}^3 (N3={n3}), LINE RSRS, x0=0, TOL=1e-10, ITMAX=500",
+                   "iterations": f["its"], "line_phase_us": f["line_us"], "line_phase_alg_GBps": f["line_GBps"],
+                   "mixed_solves_per_s": m["value"], "mixed_iterations": m["its"],
+                   "solve_alg_GBps": alg / 1e9, "solve_roofline_frac": alg / HBM_PEAK},
+        "roofline": {"bound": "hbm", "kernel": "k_line_color_sys<double> (one red-black colour of line solves)",
+                     "achieved": f["line_GBps"], "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": f["line_GBps"] * 1e9 / HBM_PEAK, "traffic": None},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line))
+    return 0
+
+
 def bench_serve(args):
     """Poisson request streams through the continuous-batching server: sides
     lognormal (median 64, sigma 0.5) clipped to [24,160], 60% upwind momentum
@@ -226,7 +286,7 @@ def main(argv=None):
     ap.add_argument("--cpu-iters", type=int, default=6, help="reference iterations timed for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
-    ap.add_argument("--workload", choices=["mixed", "batch", "serve"], default="mixed",
+    ap.add_argument("--workload", choices=["mixed", "batch", "serve", "line"], default="mixed",
                     help="mixed: the 256^3 solve (headline); batch: 64 systems 16^3..128^3 in one batched "
                          "solve; serve: Poisson request streams through the continuous-batching server")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -256,6 +316,8 @@ def main(argv=None):
         return bench_batch(args)
     if args.workload == "serve":
         return bench_serve(args)
+    if args.workload == "line":
+        return bench_line(args)
     n = args.n
     t0 = time.time()
     a, b = systems.gen_system(systems.VOID_PRESSURE, n, n, n, 2000)
@@ -309,14 +371,14 @@ def main(argv=None):
     value = args.steps / total
     fp64_value = len(fp64_times) / sum(fp64_times) if fp64_times else None
 
-    # dominant kernel: the fused first stencil pass k_p_matvec<float> (P update,
-    # V = A*(P/A0), <Rtilde,V>): 7 coefficient planes + A0 + R, P, V, Rtilde
-    # read + P, V written = 8*4 + 6*4 = 56 B/cell (HIP events in this run)
+    # dominant kernel: the first FP32 stencil pass k_matvec_dots<float, 0>
+    # (V = A*(P/A0) and <Rtilde,V>): 6 off-diagonal planes + row excess + 1/A0
+    # + operand + dot partner + result = 11*4 = 44 B/cell (HIP events in this run)
     kern_avg_s = prof["total_ms"] * 1e-3 / max(1, prof["launches"])
     kern_bytes = prof["alg_bytes"] / max(1, prof["launches"])
     achieved = kern_bytes / kern_avg_s if kern_avg_s > 0 else 0.0
     traffic = None
-    tf = ROOT / "profiles" / "p_matvec_fp32_256cubed_traffic.json"
+    tf = ROOT / "profiles" / "stencil_fp32_256cubed_traffic.json"
     if tf.exists():
         try:
             traffic = json.loads(tf.read_text()).get("bytes_per_launch")
@@ -366,7 +428,7 @@ def main(argv=None):
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_p_matvec<float> (P update + FP32 stencil A*(P/A0) + <Rtilde,V>)",
+            "kernel": "k_matvec_dots<float, 0> (FP32 stencil V = A*(P/A0) + <Rtilde,V>)",
             "achieved": achieved / 1e9,
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",

@@ -138,6 +138,15 @@ __device__ __forceinline__ auto ld_stream(P p) {
     return __builtin_nontemporal_load(p);
 }
 
+// DIAG preconditioner applied to one value.  FP64 (the reference's
+// LEQ_MSOLVE1): z = r / A0, one correctly rounded division.  FP32 inner
+// solve: the diagonal plane holds 1/A0 rounded to FP32 and z = r * (1/A0),
+// one multiply where a division costs about ten FP64 instructions; the
+// same product is used by the stencil and by the solution update, so the
+// preconditioned operator stays consistent.
+__device__ __forceinline__ double diag_apply(double r, double a0) { return r / a0; }
+__device__ __forceinline__ float diag_apply(float r, float inv_a0) { return r * inv_a0; }
+
 // Operand fetch of the stencil: x itself, or x/A0 of the neighbour (DIAG
 // preconditioner applied on the fly, LEQ_MSOLVE1 leqsol_mod.f:729-744, so
 // Phat and Shat never exist as arrays).  Both forms round exactly like the
@@ -147,7 +156,7 @@ struct Operand {
     PT x;
     PT a0;
     __device__ __forceinline__ T operator()(int64_t q) const {
-        if (DIV) return (T)((double)x[q] / (double)a0[q]);
+        if (DIV) return diag_apply(x[q], a0[q]);
         return x[q];
     }
 };

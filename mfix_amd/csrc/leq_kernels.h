@@ -46,7 +46,8 @@ struct SolverState {
     int final_update;       // mixed: the pending update ends the solve
     int y_reset;            // mixed: the FP64 correction accumulator restarts from 0
     int discard_y;          // mixed: a non-finite scalar appeared; the next update drops y
-    int fused;              // DIAG/NONE: the S update and ||S|| come with the second stencil pass
+    int restart_always;     // mixed: restart the recurrence at every reliable update (large systems)
+    int beta_invalid;       // mixed: this update cannot continue the recurrence (S-norm exit, breakdown)
     int x_cur;              // mixed: which FP64 solution buffer is current
     int restart;            // mixed: restart the recurrence (p = r, Rtilde = r) at the next iteration
     int restarts;           // mixed: recurrence restarts (one per reliable update that did not end the solve)
@@ -108,8 +109,7 @@ enum Fin : int {
     FIN_SETUP = 0,      // Rnorm0 (leq_bicgs.f:291), zero-row IER=-2 (:217-233)
     FIN_SHADOW,         // rho_0 (:318) and its zero test at i=1 (:320-332)
     FIN_ALPHA,          // alpha = rho/<Rtilde,V> (:357-361)
-    FIN_SNORM,          // ||S|| <= EPSILON exit request (:372-374)
-    FIN_OMEGA,          // omega = <T,S>/<T,T>, TxT=0 -> SMALL_NUMBER (:411-429); fused ||S|| exit
+    FIN_OMEGA,          // ||S|| <= EPSILON exit (:372-374); omega = <T,S>/<T,T>, TxT=0 -> SMALL_NUMBER (:411-429)
     FIN_XR,             // ||R||, convergence, next rho and beta (:318-343, :439-495)
     FIN_XR_MIXED,       // same for the FP32 inner recurrence, reliable-update trigger
     FIN_UPDATE,         // reliable update: FP64 true residual, convergence, restart
@@ -123,7 +123,6 @@ __device__ __forceinline__ bool fin_skips(PS st) {
     if (F == FIN_SETUP || F == FIN_DOT) return false;
     if (st->done) return true;
     if (F == FIN_OMEGA) return st->s_exit != 0;
-    if (F == FIN_SNORM) return st->fused != 0;  // decided in FIN_OMEGA
     if (F == FIN_UPDATE) return !st->need_update;
     return false;
 }
@@ -152,14 +151,9 @@ __device__ __forceinline__ void fin_apply(PS st, const double (&tot)[NV], double
     } else if (F == FIN_ALPHA) {
         S.rtv = tot[0];
         S.alpha = S.rho / tot[0];
-    } else if (F == FIN_SNORM) {
-        if (!S.minimize) {
-            S.snorm = sqrt(tot[0]);
-            if (S.snorm <= kTolMin) S.s_exit = 1;
-        }
     } else if (F == FIN_OMEGA) {
-        if (NV > 2 && S.fused && !S.minimize) {
-            // ||S|| of the fused pass (leq_bicgs.f:372-376)
+        if (NV > 2 && !S.minimize) {
+            // S-norm exit (leq_bicgs.f:372-376); ||S||^2 from the S update
             S.snorm = sqrt(tot[NV > 2 ? 2 : 0]);
             if (S.snorm <= kTolMin) {
                 S.s_exit = 1;
@@ -220,13 +214,18 @@ __device__ __forceinline__ void fin_apply(PS st, const double (&tot)[NV], double
         if (!isfinite(tot[0]) || !isfinite(tot[1]) || !isfinite(S.alpha) || !isfinite(S.omega)) {
             S.need_update = 1;
             S.discard_y = 1;
+            S.beta_invalid = 1;
         } else if (S.s_exit) {
+            // the iteration ended at its S-norm exit: omega of this iteration
+            // is stale, so the recurrence cannot continue
             S.need_update = 1;
+            S.beta_invalid = 1;
         } else {
             S.rnorm = sqrt(tot[0]);
             const double rho_new = tot[1];
             if (S.rnorm <= S.tol * S.rnorm0 || S.rnorm <= S.delta * S.upd_rnorm || rho_new == 0.0) {
                 S.need_update = 1;
+                if (rho_new == 0.0) S.beta_invalid = 1;
             } else {
                 S.beta = (rho_new / S.rho) * (S.alpha / S.omega);
                 S.rho_prev = S.rho;
@@ -240,17 +239,24 @@ __device__ __forceinline__ void fin_apply(PS st, const double (&tot)[NV], double
         }
     } else if (F == FIN_UPDATE) {
         // x <- x + y and the FP64 true residual were formed by the update
-        // kernel.  Converged on ||b - A x|| <= TOL*Rnorm0; otherwise the FP32
-        // recurrence restarts from the true residual with a fresh shadow
-        // residual (P = R, Rtilde = R, rho = <R,R>).  Continuing the old
-        // recurrence instead lets rho = <Rtilde,R> decay below FP32's rounding
-        // level on hard systems, after which alpha and omega are noise.
+        // kernel.  Converged on ||b - A x|| <= TOL*Rnorm0.  Otherwise the FP32
+        // recurrence continues from the true residual (residual replacement)
+        // or restarts from it with a fresh shadow residual (P = R,
+        // Rtilde = R, rho = <R,R>): always on large systems, where
+        // continuing let rho = <Rtilde,R> decay below FP32's rounding level
+        // in the hard phase (256^3: 6671 iterations against 1099), and on
+        // smaller ones when the update shows drift (the true residual grew
+        // since the last update or is 10x the recursive one), since there
+        // restarting every cycle costs up to 2x the iterations.
+        const bool discarded = S.discard_y;
         S.discard_y = 0;
         S.s_exit = 0;
         S.x_cur = 1 - S.x_cur;
         S.y_reset = 1;
         S.need_update = 0;
         S.updates += 1;
+        const double prev_true = S.upd_rnorm;
+        const double rec = S.rnorm;
         S.true_rnorm = sqrt(tot[0]);
         S.upd_rnorm = S.true_rnorm;
         S.rnorm = S.true_rnorm;
@@ -273,14 +279,23 @@ __device__ __forceinline__ void fin_apply(PS st, const double (&tot)[NV], double
             S.done = 1;
             return;
         }
-        S.restart = 1;
-        S.restarts += 1;
-        S.rho = tot[1];
-        if (S.rho == 0.0) {
-            S.ier = -2;
-            S.exit_reason = EXIT_RHO;
-            S.done = 1;
+        const double rho_cont = tot[1];
+        const bool drift = discarded || S.true_rnorm > prev_true || S.true_rnorm > 10.0 * rec;
+        if (S.restart_always || drift || rho_cont == 0.0 || S.beta_invalid) {
+            S.restart = 1;
+            S.restarts += 1;
+            S.rho = tot[2];
+            if (S.rho == 0.0) {
+                S.ier = -2;
+                S.exit_reason = EXIT_RHO;
+                S.done = 1;
+            }
+        } else {
+            S.beta = (rho_cont / S.rho) * (S.alpha / S.omega);
+            S.rho_prev = S.rho;
+            S.rho = rho_cont;
         }
+        S.beta_invalid = 0;
     } else if (F == FIN_DOT) {
         *out = tot[0];
     }

@@ -36,7 +36,6 @@ struct Sys {
     double tol, delta;
     const T *a;            // reference layout (FP64) or difference layout (FP32)
     T *x, *r, *rt, *p, *s, *t, *v, *phat, *shat;  // x: VAR (FP64 solve only)
-    T *p2, *v2;            // DIAG/NONE: P and V of odd iterations (P and V hold even ones)
     double *y;             // mixed: FP64 correction accumulator since the last reliable update
     T *line_scratch;       // 4 planes of DGTSV scratch (LINE)
     SolverState *st;
@@ -72,7 +71,7 @@ __global__ void k_init_states(const Sys<T> *__restrict__ sys, const int *__restr
     s.mixed = S.mixed;
     s.delta = S.delta;
     s.max_updates = S.max_updates;
-    s.fused = S.pc != PC_LINE;
+    s.restart_always = S.mixed && S.g.n3 >= (int64_t(1) << 23);
     s.true_rnorm = -1.0;
     *S.st = s;
 }
@@ -147,7 +146,6 @@ __global__ void __launch_bounds__(BS) k_shadow(const Sys<T> *__restrict__ sys, c
 template <int BS, typename T>
 __global__ void __launch_bounds__(BS) k_p_update(const Sys<T> *__restrict__ sys, const int2 *__restrict__ map) {
     SYS_BLOCK(T)
-    if (S.pc != PC_LINE) return;  // DIAG/NONE: fused into k_p_matvec
     auto *__restrict__ st = to_global(S.st);
     if (st->done) return;
     const bool restart = st->restart;
@@ -168,14 +166,15 @@ __global__ void __launch_bounds__(BS) k_p_update(const Sys<T> *__restrict__ sys,
     }
 }
 
-// LINE systems: MODE 0: V = A*Phat (leq_bicgs.f:355) and <Rtilde,V> (:357);
-// MODE 1: T = A*Shat (:409), <T,S>, <T,T> (:411-421), with Phat/Shat the
-// line-preconditioned arrays.
+// MODE 0: V = A*Phat (leq_bicgs.f:355) and <Rtilde,V> (:357).
+// MODE 1: T = A*Shat (:409), <T,S>, <T,T> (:411-421).
+// The operand is P/A0 or S/A0 for DIAG, the preconditioned array for LINE,
+// and P or S for NONE.  The S-norm exit (:372-376) is decided with omega,
+// after this pass: T is then unused, so the iterates do not change.
 template <int BS, typename T, int MODE>
 __global__ void __launch_bounds__(BS) k_matvec_dots(const Sys<T> *__restrict__ sys, const int2 *__restrict__ map) {
     __shared__ double lds[(BS / 64) * 2];
     SYS_BLOCK(T)
-    if (S.pc != PC_LINE) return;  // DIAG/NONE: k_p_matvec / k_s_matvec
     auto *__restrict__ st = to_global(S.st);
     if (st->done) return;
     if (MODE == 1 && st->s_exit) return;
@@ -208,139 +207,13 @@ __global__ void __launch_bounds__(BS) k_matvec_dots(const Sys<T> *__restrict__ s
     }
 }
 
-// DIAG/NONE systems.  P and V alternate between two buffers by iteration
-// parity, so a block can recompute its neighbours' P from the previous P
-// and V while other blocks write the new ones.
-template <typename PT>
-struct PVBuf {
-    PT cur, prev;
-};
-template <typename T>
-__device__ __forceinline__ PVBuf<gptr<T>> p_bufs(const Sys<T> &S, int i) {
-    return (i & 1) ? PVBuf<gptr<T>>{to_global(S.p2), to_global(S.p)} : PVBuf<gptr<T>>{to_global(S.p), to_global(S.p2)};
-}
-template <typename T>
-__device__ __forceinline__ PVBuf<gptr<T>> v_bufs(const Sys<T> &S, int i) {
-    return (i & 1) ? PVBuf<gptr<T>>{to_global(S.v2), to_global(S.v)} : PVBuf<gptr<T>>{to_global(S.v), to_global(S.v2)};
-}
-
-// Operand of the first stencil: P_i = R (first/restart) or R + beta*(P - omega*V)
-// (leq_bicgs.f:334-343) evaluated where it is needed, divided by A0 for DIAG
-// (LEQ_MSOLVE1): the same operations as the separate passes, so the same bits.
-template <typename T, bool DIV>
-struct POperand {
-    gptr<const T> r, p, v, a0;
-    double beta, omega;
-    bool first;
-    __device__ __forceinline__ T pval(int64_t q) const {
-        if (first) return r[q];
-        return (T)((double)r[q] + beta * ((double)p[q] - omega * (double)v[q]));
-    }
-    __device__ __forceinline__ T operator()(int64_t q) const {
-        const T pv = pval(q);
-        if (DIV) return (T)((double)pv / (double)a0[q]);
-        return pv;
-    }
-};
-
-// Operand of the second stencil: S = R - alpha*V (leq_bicgs.f:365-367), /A0 for DIAG.
-template <typename T, bool DIV>
-struct SOperand {
-    gptr<const T> r, v, a0;
-    double alpha;
-    __device__ __forceinline__ T sval(int64_t q) const { return (T)((double)r[q] - alpha * (double)v[q]); }
-    __device__ __forceinline__ T operator()(int64_t q) const {
-        const T sv = sval(q);
-        if (DIV) return (T)((double)sv / (double)a0[q]);
-        return sv;
-    }
-};
-
-// P update, V = A*Phat and <Rtilde,V> in one pass (DIAG/NONE systems).
-template <int BS, typename T>
-__global__ void __launch_bounds__(BS) k_p_matvec(const Sys<T> *__restrict__ sys, const int2 *__restrict__ map) {
-    __shared__ double lds[(BS / 64) * 1];
-    SYS_BLOCK(T)
-    if (S.pc == PC_LINE) return;
-    auto *__restrict__ st = to_global(S.st);
-    if (st->done) return;
-    const int it = st->i;
-    const bool restart = st->restart;
-    const PVBuf<gptr<T>> pb = p_bufs(S, it), vb = v_bufs(S, it);
-    auto *__restrict__ a = to_global(S.a);
-    const auto a0 = a + (int64_t)DiagPlane<T>::value * g.n3;
-    auto *__restrict__ r = to_global(S.r);
-    auto *__restrict__ rt = to_global(S.rt);
-    double v[1] = {0.0};
-    auto body = [&](auto X) {
-        FOR_CELLS(BS, P, blk, c) {
-            const Nb n = neighbours_fast(g, c);
-            const T yy = Stencil<T>::row(g, a, X, c, n);
-            pb.cur[c] = X.pval(c);
-            vb.cur[c] = yy;
-            T wv;
-            if (restart) {
-                wv = r[c];
-                rt[c] = wv;
-            } else {
-                wv = rt[c];
-            }
-            if (g.dot_all || n.interior) v[0] += (double)wv * (double)yy;
-        }
-    };
-    const bool first = it == 1 || restart;
-    if (S.pc == PC_DIAG)
-        body(POperand<T, true>{r, pb.prev, vb.prev, a0, st->beta, st->omega, first});
-    else
-        body(POperand<T, false>{r, pb.prev, vb.prev, a0, st->beta, st->omega, first});
-    block_partial<BS, 1>(v, to_global(S.partials), P.nblk, blk, lds);
-}
-
-// S update, T = A*Shat, <T,S>, <T,T> and ||S||^2 in one pass (DIAG/NONE).
-// The S-norm exit (leq_bicgs.f:372-376) is decided with the other scalars;
-// T is then unused, so the iterates are those of the separate passes.
-template <int BS, typename T>
-__global__ void __launch_bounds__(BS) k_s_matvec(const Sys<T> *__restrict__ sys, const int2 *__restrict__ map) {
-    __shared__ double lds[(BS / 64) * 3];
-    SYS_BLOCK(T)
-    if (S.pc == PC_LINE) return;
-    auto *__restrict__ st = to_global(S.st);
-    if (st->done) return;
-    const PVBuf<gptr<T>> vb = v_bufs(S, st->i);
-    auto *__restrict__ a = to_global(S.a);
-    const auto a0 = a + (int64_t)DiagPlane<T>::value * g.n3;
-    auto *__restrict__ r = to_global(S.r);
-    auto *__restrict__ s = to_global(S.s);
-    auto *__restrict__ t = to_global(S.t);
-    const bool all = g.dot_all && !st->minimize;
-    double v[3] = {0.0, 0.0, 0.0};
-    auto body = [&](auto X) {
-        FOR_CELLS(BS, P, blk, c) {
-            const Nb n = neighbours_fast(g, c);
-            const T yy = Stencil<T>::row(g, a, X, c, n);
-            const T sv = X.sval(c);
-            s[c] = sv;
-            t[c] = yy;
-            const double yd = (double)yy, sd = (double)sv;
-            if (all || n.interior) {
-                v[0] += yd * sd;
-                v[1] += yd * yd;
-            }
-            if (g.dot_all || n.interior) v[2] += sd * sd;
-        }
-    };
-    if (S.pc == PC_DIAG) body(SOperand<T, true>{r, vb.cur, a0, st->alpha});
-    else body(SOperand<T, false>{r, vb.cur, a0, st->alpha});
-    block_partial<BS, 3>(v, to_global(S.partials), P.nblk, blk, lds);
-}
-
 // S = R - alpha*V (leq_bicgs.f:365-367) and ||S||^2 (:373); for LINE also
-// the preconditioner's start value Shat = S.
+// the preconditioner's start value Shat = S.  The S-norm exit test is applied
+// by the finish after the second stencil pass.
 template <int BS, typename T>
 __global__ void __launch_bounds__(BS) k_s_update(const Sys<T> *__restrict__ sys, const int2 *__restrict__ map) {
     __shared__ double lds[(BS / 64) * 1];
     SYS_BLOCK(T)
-    if (S.pc != PC_LINE) return;  // DIAG/NONE: fused into k_s_matvec
     auto *__restrict__ st = to_global(S.st);
     if (st->done) return;
     double acc[1] = {0.0};
@@ -355,7 +228,9 @@ __global__ void __launch_bounds__(BS) k_s_update(const Sys<T> *__restrict__ sys,
         if (sh) sh[c] = sv;
         if (g.dot_all || is_interior(g, c)) acc[0] += (double)sv * (double)sv;
     }
-    block_partial<BS, 1>(acc, to_global(S.partials), P.nblk, blk, lds);
+    // slot 2: the second stencil pass writes slots 0 and 1, and one finish
+    // launch after it takes all three
+    block_partial<BS, 1>(acc, to_global(S.partials) + 2 * P.nblk, P.nblk, blk, lds);
 }
 
 // Var += alpha*Phat + omega*Shat; R = S - omega*T (leq_bicgs.f:432-436);
@@ -375,7 +250,7 @@ __global__ void __launch_bounds__(BS) k_xr_update(const Sys<T> *__restrict__ sys
     const bool sexit = st->s_exit;
     const bool yr = st->y_reset;
     const bool div = S.pc == PC_DIAG;
-    const auto ph = S.pc == PC_LINE ? to_global(S.phat) : p_bufs(S, st->i).cur;
+    const auto ph = S.pc == PC_LINE ? to_global(S.phat) : to_global(S.p);
     const auto sh = S.pc == PC_LINE ? to_global(S.shat) : to_global(S.s);
     auto *__restrict__ a0 = to_global(S.a) + (int64_t)DiagPlane<T>::value * g.n3;
     auto *__restrict__ x = to_global(S.x);
@@ -387,10 +262,11 @@ __global__ void __launch_bounds__(BS) k_xr_update(const Sys<T> *__restrict__ sys
     auto *__restrict__ rt = to_global(S.rt);
     FOR_CELLS(BS, P, blk, c) {
         const double x0 = mixed ? (yr ? 0.0 : y[c]) : (double)x[c];
-        double phv, d0 = 1.0;
+        double phv;
+        T d0 = (T)1;
         if (div) {
-            d0 = (double)a0[c];
-            phv = (double)(T)((double)ph[c] / d0);
+            d0 = a0[c];
+            phv = (double)diag_apply((T)ph[c], d0);
         } else {
             phv = (double)ph[c];
         }
@@ -398,7 +274,7 @@ __global__ void __launch_bounds__(BS) k_xr_update(const Sys<T> *__restrict__ sys
             if (mixed) y[c] = x0 + alpha * phv;
             else x[c] = (T)(x0 + alpha * phv);
         } else {
-            const double shv = div ? (double)(T)((double)sh[c] / d0) : (double)sh[c];
+            const double shv = div ? (double)diag_apply((T)sh[c], d0) : (double)sh[c];
             if (mixed) y[c] = x0 + alpha * phv + omega * shv;
             else x[c] = (T)(x0 + alpha * phv + omega * shv);
             const T rv = (T)((double)s[c] - omega * (double)t[c]);
@@ -414,13 +290,13 @@ __global__ void __launch_bounds__(BS) k_xr_update(const Sys<T> *__restrict__ sys
 
 // Reliable update (mixed path): xo = xi + y, r64 = b - A64*(xi + y) with the
 // neighbours' xi + y formed on the fly (xi is read-only here, xo is the other
-// buffer), r = (float)r64; partial ||r64||^2 and <r, r> (rho of the restarted
-// recurrence, whose Rtilde is r).  After a breakdown (discard_y) y is
-// dropped: xo = xi.
+// buffer), r = (float)r64; partial ||r64||^2, <Rtilde, r> (rho of the
+// continued recurrence) and <r, r> (rho of a restarted one, whose Rtilde is
+// r).  After a breakdown (discard_y) y is dropped: xo = xi.
 template <int BS>
 __global__ void __launch_bounds__(BS) k_reliable_update(const Sys<float> *__restrict__ sys,
                                                         const int2 *__restrict__ map) {
-    __shared__ double lds[(BS / 64) * 2];
+    __shared__ double lds[(BS / 64) * 3];
     SYS_BLOCK(float)
     auto *__restrict__ st = to_global(S.st);
     if (st->done || !st->need_update) return;
@@ -430,7 +306,8 @@ __global__ void __launch_bounds__(BS) k_reliable_update(const Sys<float> *__rest
     auto *__restrict__ a = to_global(S.a64);
     auto *__restrict__ b = to_global(S.b64);
     auto *__restrict__ r = to_global(S.r);
-    double acc[2] = {0.0, 0.0};
+    auto *__restrict__ rt = to_global(S.rt);
+    double acc[3] = {0.0, 0.0, 0.0};
     struct XY {
         gptr<const double> xi;
         gptr<const double> y;
@@ -444,14 +321,16 @@ __global__ void __launch_bounds__(BS) k_reliable_update(const Sys<float> *__rest
         r[c] = r32;
         if (g.dot_all || n.interior) {
             acc[0] += rr * rr;
-            acc[1] += (double)r32 * (double)r32;
+            acc[1] += (double)rt[c] * (double)r32;
+            acc[2] += (double)r32 * (double)r32;
         }
     }
-    block_partial<BS, 2>(acc, to_global(S.partials), P.nblk, blk, lds);
+    block_partial<BS, 3>(acc, to_global(S.partials), P.nblk, blk, lds);
 }
 
 // FP32 difference-layout copy of the (scaled) FP64 matrix: off-diagonals
-// rounded, plane 3 = A0 + sum(off-diagonals) formed in FP64, plane 7 = A0.
+// rounded, plane 3 = A0 + sum(off-diagonals) formed in FP64, plane 7 = 1/A0
+// (DIAG) or A0 (LINE, NONE).
 // A coupling clamped to the cell itself contributes a_l*(x_c - x_c) = 0 in
 // the difference form and a_l*x_c through the excess, as in LEQ_MATVEC.
 template <int BS>
@@ -471,7 +350,9 @@ __global__ void __launch_bounds__(BS) k_prepare_a32(const Sys<float> *__restrict
             e += v;
         }
         a32[3 * n3 + c] = (float)e;
-        a32[7 * n3 + c] = (float)a[3 * n3 + c];
+        // DIAG systems keep 1/A0 (the inner DIAG preconditioner multiplies),
+        // LINE systems A0 (the line solves' diagonal)
+        a32[7 * n3 + c] = S.pc == PC_DIAG ? (float)(1.0 / a[3 * n3 + c]) : (float)a[3 * n3 + c];
     }
 }
 
@@ -493,24 +374,25 @@ __global__ void __launch_bounds__(kFinishBlock) k_finish_sys(const Sys<T> *__res
 
 // Red-black line phase along J for every LINE system of a table (RSRS and
 // the 2-D R sweep).  Block b takes lines [map[b].y*64, +64) of colour
-// `param` of system map[b].x, numbered densely by color_line; lane l owns
-// line l of the block and runs DGTSV on it (same operations and order as
-// solve_line, so the result is bit-identical).  Lines along J are contiguous,
-// so the block reads rows in tiles of 8: element k of a lane is row
-// (lane & 7) of line (lane >> 3) + 8k, and each load instruction fetches 8
-// lines x 64 contiguous bytes instead of 64 scattered words.  The right-hand
-// side (B minus the I/K neighbour terms, whose offsets are constant along a
-// J-line) is formed as the tile is loaded, and the tile is transposed through
-// LDS to the owning lanes.  The next tile's loads are in flight while the
-// current tile is eliminated.  Finished rows go to scratch in [row][lane]
-// order (coalesced), are read back in groups of 8 rows by the back solve, and
-// the solution is transposed through LDS again for coalesced stores.  which
-// = 0 preconditions P into Phat, 1 S into Shat (skipped after the S-norm exit).
+// `param` of system map[b].x, numbered densely by color_line, with two
+// waves: lane l of wave 0 runs DGTSV on line l (the operations and order of
+// solve_line, so the result is bit-identical); wave 1 feeds it.  Lines along
+// J are contiguous, so wave 1 reads rows in tiles of 8 - element k of its
+// lane is row (lane & 7) of line (lane >> 3) + 8k, each load instruction
+// fetching 8 lines x 64 contiguous bytes - forms the right-hand side (B
+// minus the I/K neighbour terms, whose offsets are constant along a J-line)
+// and stages the tile in LDS while wave 0 eliminates the previous tile.
+// Finished rows go to scratch in [row][lane] order; the back solve reads them
+// in groups of 8 rows, the next group's loads in flight while the current
+// one is solved, and wave 1 writes the solution out through LDS, coalesced.
+// which = 0 preconditions P into Phat, 1 S into Shat.
 template <typename T>
-__global__ void __launch_bounds__(64) k_line_color_sys(const Sys<T> *__restrict__ sys, const int2 *__restrict__ map,
-                                                       int param, int which) {
+__global__ void __launch_bounds__(128) k_line_color_sys(const Sys<T> *__restrict__ sys, const int2 *__restrict__ map,
+                                                        int param, int which) {
     constexpr int R = 8;
     __shared__ T tl[2][4][R][64];  // dl, d, du, bb: R rows x 64 lines, double buffered
+    __shared__ T xs[R][64];        // solution rows staged for the stores
+    __shared__ int linfo[64];      // DGTSV INFO per line
     const int2 e = map[blockIdx.x];
     if (e.x < 0) return;
     const Sys<T> &S = sys[e.x];
@@ -520,11 +402,10 @@ __global__ void __launch_bounds__(64) k_line_color_sys(const Sys<T> *__restrict_
     const LineGeom L = line_geom(g, AX_J);
     const int n = L.n;
     const int64_t n3 = g.n3;
-    const int lane = threadIdx.x;
+    const int wave = threadIdx.x >> 6;  // 0: elimination and back solve, 1: loads and stores
+    const int lane = threadIdx.x & 63;
     const int do_k = g.do_k;
     constexpr int pd = DiagPlane<T>::value;
-    // uniform plane bases in the global address space; cells are addressed
-    // with 32-bit offsets (N3 < 2^31)
     const gptr<const T> a = to_global(S.a);
     const gptr<const T> pI_m = a + 2 * n3, pI_p = a + 4 * n3, pK_m = a, pK_p = a + 6 * n3;
     const gptr<const T> pD = a + pd * n3, pJ_m = a + n3, pJ_p = a + 5 * n3;
@@ -536,7 +417,7 @@ __global__ void __launch_bounds__(64) k_line_color_sys(const Sys<T> *__restrict_
     int u = 0, v = 0;
     const int vld = color_line(L.nu, L.nv, do_k ? 1 : 0, param, (int64_t)e.y * 64 + lane, u, v) ? 1 : 0;
     const int rsub = lane & 7;
-    // loader view: element k of this lane is row rsub of line (lane >> 3) + 8k,
+    // loader view (wave 1): element k is row rsub of line (lane >> 3) + 8k,
     // whose start cell and I/K neighbour offsets (constant along a J-line,
     // set_increments.f:72-112) are kept per element
     uint32_t jc[R], jim[R], jip[R], jkm[R], jkp[R];
@@ -576,104 +457,145 @@ __global__ void __launch_bounds__(64) k_line_color_sys(const Sys<T> *__restrict_
         tl[BUF][3][rsub][j] = bb;                                                                        \
     }
     const int ntiles = (n + R - 1) / R;
-    MFIX_LINE_TILE(0, 0)
+    if (wave == 1) {
+        MFIX_LINE_TILE(0, 0)
+    }
     __syncthreads();
     T d0 = (T)0, du0 = (T)0, b0 = (T)0;
     int info = 0;
     for (int tt = 0; tt < ntiles; ++tt) {
         const int buf = tt & 1;
-        const int q0 = tt * R;
-        // DGTSV elimination of this lane's line over the tile (DGTSV.f:107-163,
-        // the operations of solve_line in the same order)
-        for (int r = 0; r < R; ++r) {
-            const int q = q0 + r;
-            if (q >= n) break;
-            const T dl1 = tl[buf][0][r][lane];
-            T d1 = tl[buf][1][r][lane], du1 = tl[buf][2][r][lane], b1 = tl[buf][3][r][lane];
-            if (q == 0) {
+        if (wave == 1) {
+            if (tt + 1 < ntiles) {
+                if (buf) {
+                    MFIX_LINE_TILE((tt + 1) * R, 0)
+                } else {
+                    MFIX_LINE_TILE((tt + 1) * R, 1)
+                }
+            }
+        } else {
+            // DGTSV elimination of this lane's line over the tile
+            // (DGTSV.f:107-163, the operations of solve_line in the same order)
+            const int q0 = tt * R;
+            for (int r = 0; r < R; ++r) {
+                const int q = q0 + r;
+                if (q >= n) break;
+                const T dl1 = tl[buf][0][r][lane];
+                T d1 = tl[buf][1][r][lane], du1 = tl[buf][2][r][lane], b1 = tl[buf][3][r][lane];
+                if (q == 0) {
+                    d0 = d1;
+                    du0 = du1;
+                    b0 = b1;
+                    continue;
+                }
+                const int i = q - 1;
+                T fd, fu, fu2, fb;
+                if (fabs((double)d0) >= fabs((double)dl1)) {
+                    if (d0 == (T)0 && !info) info = i + 1;
+                    const T fact = dl1 / d0;
+                    d1 = d1 - fact * du0;
+                    b1 = b1 - fact * b0;
+                    fd = d0;
+                    fu = du0;
+                    fu2 = (T)0;
+                    fb = b0;
+                } else {
+                    const T fact = d0 / dl1;
+                    fd = dl1;
+                    T temp = d1;
+                    d1 = du0 - fact * temp;
+                    if (i < n - 2) {
+                        fu2 = du1;
+                        du1 = -fact * fu2;
+                    } else {
+                        fu2 = (T)0;
+                    }
+                    fu = temp;
+                    temp = b0;
+                    fb = b1;
+                    b1 = temp - fact * b1;
+                }
+                const uint32_t o = (uint32_t)i * 64 + lane;
+                sc[o] = fd;
+                sc[pl + o] = fu;
+                sc[2 * pl + o] = fu2;
+                sc[3 * pl + o] = fb;
                 d0 = d1;
                 du0 = du1;
                 b0 = b1;
-                continue;
-            }
-            const int i = q - 1;
-            T fd, fu, fu2, fb;
-            if (fabs((double)d0) >= fabs((double)dl1)) {
-                if (d0 == (T)0 && !info) info = i + 1;
-                const T fact = dl1 / d0;
-                d1 = d1 - fact * du0;
-                b1 = b1 - fact * b0;
-                fd = d0;
-                fu = du0;
-                fu2 = (T)0;
-                fb = b0;
-            } else {
-                const T fact = d0 / dl1;
-                fd = dl1;
-                T temp = d1;
-                d1 = du0 - fact * temp;
-                if (i < n - 2) {
-                    fu2 = du1;
-                    du1 = -fact * fu2;
-                } else {
-                    fu2 = (T)0;
-                }
-                fu = temp;
-                temp = b0;
-                fb = b1;
-                b1 = temp - fact * b1;
-            }
-            const uint32_t o = (uint32_t)i * 64 + lane;
-            sc[o] = fd;
-            sc[pl + o] = fu;
-            sc[2 * pl + o] = fu2;
-            sc[3 * pl + o] = fb;
-            d0 = d1;
-            du0 = du1;
-            b0 = b1;
-        }
-        if (tt + 1 < ntiles) {
-            if (buf) {
-                MFIX_LINE_TILE((tt + 1) * R, 0)
-            } else {
-                MFIX_LINE_TILE((tt + 1) * R, 1)
             }
         }
         __syncthreads();
     }
 #undef MFIX_LINE_TILE
-    if (!info && d0 == (T)0) info = n;  // DGTSV.f:160-163
+    if (wave == 0) {
+        if (!info && d0 == (T)0) info = n;  // DGTSV.f:160-163
+        linfo[lane] = info;
+    }
+    __syncthreads();
     int jinfo[R];
 #pragma unroll
-    for (int k = 0; k < R; ++k) jinfo[k] = __shfl(info, (lane >> 3) + 8 * k, 64);
-    // back solve (DGTSV.f:233-246) in groups of R rows from the top row down;
-    // each group is transposed through LDS for coalesced stores of the line
+    for (int k = 0; k < R; ++k) jinfo[k] = linfo[(lane >> 3) + 8 * k];
+    // back solve (DGTSV.f:233-246) in groups of R rows from the top row
+    // down; rows n-2 .. 0 come from scratch (row n-1 is still in registers)
     T x1 = b0 / d0, x2 = (T)0;
+    T gD[R], gU[R], gU2[R], gB[R];
+    auto fetch = [&](int hi) __attribute__((always_inline)) {
+        const int lo = hi - R + 1 > 0 ? hi - R + 1 : 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int q = hi - r;
+            if (q >= lo && q <= n - 2) {
+                const uint32_t o = (uint32_t)q * 64 + lane;
+                gD[r] = sc[o];
+                gU[r] = sc[pl + o];
+                gU2[r] = sc[2 * pl + o];
+                gB[r] = sc[3 * pl + o];
+            }
+        }
+    };
+    if (wave == 0) fetch(n - 1);
     for (int hi = n - 1; hi >= 0; hi -= R) {
         const int lo = hi - R + 1 > 0 ? hi - R + 1 : 0;
-        for (int q = hi; q >= lo; --q) {
-            T x;
-            const uint32_t o = (uint32_t)q * 64 + lane;
-            if (q == n - 1) {
-                x = x1;
-            } else if (q == n - 2) {
-                x = (sc[3 * pl + o] - sc[pl + o] * x1) / sc[o];
-                x2 = x1;
-                x1 = x;
-            } else {
-                x = (sc[3 * pl + o] - sc[pl + o] * x1 - sc[2 * pl + o] * x2) / sc[o];
-                x2 = x1;
-                x1 = x;
+        if (wave == 0) {
+            T cD[R], cU[R], cU2[R], cB[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                cD[r] = gD[r];
+                cU[r] = gU[r];
+                cU2[r] = gU2[r];
+                cB[r] = gB[r];
             }
-            tl[0][0][q - lo][lane] = x;
+            if (lo > 0) fetch(lo - 1);  // next group in flight while this one is solved
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int q = hi - r;
+                if (q >= lo) {
+                    T x;
+                    if (q == n - 1) {
+                        x = x1;
+                    } else if (q == n - 2) {
+                        x = (cB[r] - cU[r] * x1) / cD[r];
+                        x2 = x1;
+                        x1 = x;
+                    } else {
+                        x = (cB[r] - cU[r] * x1 - cU2[r] * x2) / cD[r];
+                        x2 = x1;
+                        x1 = x;
+                    }
+                    xs[q - lo][lane] = x;
+                }
+            }
         }
         __syncthreads();
+        if (wave == 1) {
 #pragma unroll
-        for (int k = 0; k < R; ++k) {
-            const int q = lo + rsub;
-            // a line whose elimination hit a zero pivot keeps its values
-            // (leqsol_mod.f:910-914)
-            if (jok[k] && !jinfo[k] && q <= hi) var[jc[k] + (uint32_t)q] = tl[0][0][rsub][(lane >> 3) + 8 * k];
+            for (int k = 0; k < R; ++k) {
+                const int q = lo + rsub;
+                // a line whose elimination hit a zero pivot keeps its values
+                // (leqsol_mod.f:910-914)
+                if (jok[k] && !jinfo[k] && q <= hi) var[jc[k] + (uint32_t)q] = xs[rsub][(lane >> 3) + 8 * k];
+            }
         }
         __syncthreads();
     }

@@ -303,8 +303,7 @@ double alg_bytes_per_cell(int pc, int sa, int sv, int sx) {
 
 static inline size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
-// FP64: R, Rtilde, P, S, T, V (+ Phat, Shat and the DGTSV scratch for LINE,
-// a second P and V for DIAG/NONE, whose fused passes alternate buffers).
+// FP64: R, Rtilde, P, S, T, V (+ Phat, Shat and the DGTSV scratch for LINE).
 // Mixed: the FP32 r, rt, p, s, t, v (+ LINE arrays), the FP64 correction
 // accumulator y, the 8-plane FP32 matrix, the FP64 residual and second
 // solution buffer.  Both: the
@@ -315,7 +314,6 @@ size_t sys_bytes(const DGrid &g, const Part &P, int pc, bool mixed) {
     const bool line = pc == PC_LINE;
     size_t b = 0;
     b += (size_t)6 * align256(n3 * sv);
-    if (!line) b += (size_t)2 * align256(n3 * sv);  // second P and V buffers
     if (mixed) b += align256(n3 * sizeof(double));  // y
     if (line) b += 2 * align256(n3 * sv) + align256((size_t)line_scratch_elems(g) * sv);
     if (mixed) b += align256(8 * n3 * sizeof(float)) + 2 * align256(n3 * sizeof(double));
@@ -358,13 +356,6 @@ void sys_carve(Sys<T> &S, const SysDesc &d, const Part &P, char *base) {
     } else {
         S.x = (T *)d.x;
         S.y = nullptr;
-    }
-    if (d.pc != PC_LINE) {
-        S.p2 = (T *)take(n3 * sizeof(T));
-        S.v2 = (T *)take(n3 * sizeof(T));
-    } else {
-        S.p2 = S.p;
-        S.v2 = S.v;
     }
     if (d.pc == PC_LINE) {
         S.phat = (T *)take(n3 * sizeof(T));
@@ -417,16 +408,13 @@ int build_maps(const Group<T> &G, const std::vector<int> &ids, MapSet &M, hipStr
     for (int id : ids) {
         const Sys<T> &S = G.sys[id];
         cells += (double)S.g.n3;
-        // first stencil pass: 7 coefficient planes (+ A0 for DIAG), and for
-        // the fused DIAG/NONE pass R, P, V, Rtilde read and P, V written;
-        // for LINE Phat, Rtilde read and V written
-        if (S.pc == PC_LINE) {
-            ++n_line;
-            k1 += (double)S.g.n3 * (7.0 * sa + 3.0 * sv);
-        } else {
-            ++n_fused;
-            k1 += (double)S.g.n3 * ((S.pc == PC_DIAG ? 8.0 : 7.0) * sa + 6.0 * sv);
-        }
+        // first stencil pass: the coefficient planes it streams (7; the FP32
+        // difference layout of a DIAG system adds its 1/A0 plane), the
+        // operand, the dot partner and the result
+        if (S.pc == PC_LINE) ++n_line;
+        else ++n_fused;
+        const double planes = (S.pc == PC_DIAG && sizeof(T) == sizeof(float)) ? 8.0 : 7.0;
+        k1 += (double)S.g.n3 * (planes * sa + 3.0 * sv);
         // stream kernels: an 8-aligned range whose slot j holds the logical
         // block that puts one contiguous slab of the system on each XCD
         const int64_t nb = S.P.nblk;
@@ -554,7 +542,7 @@ static void line_sweeps(const Group<T> &G, const MapSet &M, int which, hipStream
         if (M.lmap[q].empty()) continue;
         for (int param : {1, 0}) {
             pf.begin(s);
-            hipLaunchKernelGGL(k_line_color_sys<T>, dim3((unsigned)M.lmap[q].size()), dim3(64), 0, s, G.d_sys,
+            hipLaunchKernelGGL(k_line_color_sys<T>, dim3((unsigned)M.lmap[q].size()), dim3(128), 0, s, G.d_sys,
                                M.d_lmap[q], param, which);
             pf.end(s, 0.5 * M.cells * 13.0 * sizeof(T));
         }
@@ -568,37 +556,27 @@ static void line_sweeps(const Group<T> &G, const MapSet &M, int which, hipStream
     }
 }
 
-// One BiCGSTAB iteration (leq_bicgs.f:316-466) over the table.  DIAG/NONE
-// systems take the fused passes (P update + first stencil, S update +
-// second stencil); LINE systems the separate passes around the line sweeps.
-// A table with only one kind skips the other kind's launches.
+// One BiCGSTAB iteration (leq_bicgs.f:316-466) over the table.
 template <int BS, typename T>
 static void iteration_bs(const Group<T> &G, const MapSet &M, hipStream_t s) {
     const dim3 gs((unsigned)M.smap.size()), blk(BS);
     Prof &pf = ctx().prof[0];
-    const bool line = M.n_line > 0, fused = M.n_fused > 0;
+    const bool line = M.n_line > 0;
+    hipLaunchKernelGGL((k_p_update<BS, T>), gs, blk, 0, s, G.d_sys, M.d_smap);
+    if (line) line_sweeps(G, M, 0, s);
     pf.begin(s);
-    if (fused) hipLaunchKernelGGL((k_p_matvec<BS, T>), gs, blk, 0, s, G.d_sys, M.d_smap);
-    if (line) {
-        hipLaunchKernelGGL((k_p_update<BS, T>), gs, blk, 0, s, G.d_sys, M.d_smap);
-        line_sweeps(G, M, 0, s);
-        hipLaunchKernelGGL((k_matvec_dots<BS, T, 0>), gs, blk, 0, s, G.d_sys, M.d_smap);
-    }
+    hipLaunchKernelGGL((k_matvec_dots<BS, T, 0>), gs, blk, 0, s, G.d_sys, M.d_smap);
     pf.end(s, M.k1_bytes);
     finish<T, FIN_ALPHA, 1>(G, M, s);
-    if (fused) hipLaunchKernelGGL((k_s_matvec<BS, T>), gs, blk, 0, s, G.d_sys, M.d_smap);
-    if (line) {
-        hipLaunchKernelGGL((k_s_update<BS, T>), gs, blk, 0, s, G.d_sys, M.d_smap);
-        finish<T, FIN_SNORM, 1>(G, M, s);
-        line_sweeps(G, M, 1, s);
-        hipLaunchKernelGGL((k_matvec_dots<BS, T, 1>), gs, blk, 0, s, G.d_sys, M.d_smap);
-    }
+    hipLaunchKernelGGL((k_s_update<BS, T>), gs, blk, 0, s, G.d_sys, M.d_smap);
+    if (line) line_sweeps(G, M, 1, s);
+    hipLaunchKernelGGL((k_matvec_dots<BS, T, 1>), gs, blk, 0, s, G.d_sys, M.d_smap);
     finish<T, FIN_OMEGA, 3>(G, M, s);
     hipLaunchKernelGGL((k_xr_update<BS, T>), gs, blk, 0, s, G.d_sys, M.d_smap);
     if (G.mixed) {
         finish<T, FIN_XR_MIXED, 2>(G, M, s);
         hipLaunchKernelGGL((k_reliable_update<BS>), gs, blk, 0, s, (const Sys<float> *)G.d_sys, M.d_smap);
-        finish<T, FIN_UPDATE, 2>(G, M, s);
+        finish<T, FIN_UPDATE, 3>(G, M, s);
     } else {
         finish<T, FIN_XR, 2>(G, M, s);
     }
