@@ -242,12 +242,13 @@ __device__ __forceinline__ void fin_apply(PS st, const double (&tot)[NV], double
         // kernel.  Converged on ||b - A x|| <= TOL*Rnorm0.  Otherwise the FP32
         // recurrence continues from the true residual (residual replacement)
         // or restarts from it with a fresh shadow residual (P = R,
-        // Rtilde = R, rho = <R,R>): always on large systems, where
-        // continuing let rho = <Rtilde,R> decay below FP32's rounding level
-        // in the hard phase (256^3: 6671 iterations against 1099), and on
-        // smaller ones when the update shows drift (the true residual grew
-        // since the last update or is 10x the recursive one), since there
-        // restarting every cycle costs up to 2x the iterations.
+        // Rtilde = R, rho = <R,R>): always on systems of 2^18 cells and
+        // more, where continuing lets rho = <Rtilde,R> decay below FP32's
+        // rounding level in the hard phase (256^3: 6671 iterations against
+        // 1099; 90^3: 1632 against 995), and on smaller ones only when the
+        // update shows drift (the true residual grew since the last update
+        // or is 10x the recursive one), since there restarting every cycle
+        // costs up to 2x the iterations (32^3: 560 against 349).
         const bool discarded = S.discard_y;
         S.discard_y = 0;
         S.s_exit = 0;

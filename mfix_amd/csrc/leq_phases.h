@@ -71,7 +71,7 @@ __global__ void k_init_states(const Sys<T> *__restrict__ sys, const int *__restr
     s.mixed = S.mixed;
     s.delta = S.delta;
     s.max_updates = S.max_updates;
-    s.restart_always = S.mixed && S.g.n3 >= (int64_t(1) << 23);
+    s.restart_always = S.mixed && S.g.n3 >= (int64_t(1) << 18);
     s.true_rnorm = -1.0;
     *S.st = s;
 }
@@ -433,29 +433,42 @@ __global__ void __launch_bounds__(128) k_line_color_sys(const Sys<T> *__restrict
         jkm[k] = do_k ? (uint32_t)(nb_minus(kv, g.nk, g.cyc_z) - kv) * c2 : 0u;
         jkp[k] = do_k ? (uint32_t)(nb_plus(kv, g.nk, g.cyc_z) - kv) * c2 : 0u;
     }
-    // Load tile rows [Q0, Q0+R) and stage them in LDS buffer BUF: the
-    // right-hand side in LEQ_IKSWEEP's order (leqsol_mod.f:891-902).
-#define MFIX_LINE_TILE(Q0, BUF)                                                                          \
+    // Load tile rows [Q0, Q0+R) (every load of the tile issued before any is
+    // used) and stage them in LDS buffer BUF: the right-hand side in
+    // LEQ_IKSWEEP's order (leqsol_mod.f:891-902).
+    T tb[R], t1m[R], t1p[R], t2m[R], t2p[R], v1m[R], v1p[R], v2m[R], v2p[R], td[R], tdl[R], tdu[R];
+#define MFIX_LINE_LOAD(Q0)                                                                               \
     _Pragma("unroll") for (int k = 0; k < R; ++k) {                                                      \
         const int q = (Q0) + rsub;                                                                       \
-        T bb = (T)0, dd = (T)0, dl = (T)0, du = (T)0;                                                    \
-        if (jok[k] && q < n) {                                                                           \
-            const uint32_t c = jc[k] + (uint32_t)q;                                                      \
-            if (do_k)                                                                                    \
-                bb = bm[c] - pI_m[c] * var[c + jim[k]] - pI_p[c] * var[c + jip[k]] -                     \
-                     pK_m[c] * var[c + jkm[k]] - pK_p[c] * var[c + jkp[k]];                              \
-            else                                                                                         \
-                bb = bm[c] - pI_m[c] * var[c + jim[k]] - pI_p[c] * var[c + jip[k]];                      \
-            dd = pD[c];                                                                                  \
-            dl = q == 0 ? (T)0 : pJ_m[c];                                                                \
-            du = q == n - 1 ? (T)0 : pJ_p[c];                                                            \
-        }                                                                                                \
+        const bool ok = jok[k] && q < n;                                                                 \
+        const uint32_t c = ok ? jc[k] + (uint32_t)q : jc[k]; /* in bounds even if loaded */                \
+        tb[k] = ok ? bm[c] : (T)0;                                                                       \
+        t1m[k] = ok ? pI_m[c] : (T)0;                                                                    \
+        t1p[k] = ok ? pI_p[c] : (T)0;                                                                    \
+        v1m[k] = ok ? var[c + jim[k]] : (T)0;                                                            \
+        v1p[k] = ok ? var[c + jip[k]] : (T)0;                                                            \
+        t2m[k] = ok && do_k ? pK_m[c] : (T)0;                                                            \
+        t2p[k] = ok && do_k ? pK_p[c] : (T)0;                                                            \
+        v2m[k] = ok && do_k ? var[c + jkm[k]] : (T)0;                                                    \
+        v2p[k] = ok && do_k ? var[c + jkp[k]] : (T)0;                                                    \
+        td[k] = ok ? pD[c] : (T)0;                                                                       \
+        tdl[k] = ok && q > 0 ? pJ_m[c] : (T)0;                                                           \
+        tdu[k] = ok && q < n - 1 ? pJ_p[c] : (T)0;                                                       \
+    }
+#define MFIX_LINE_STAGE(BUF)                                                                             \
+    _Pragma("unroll") for (int k = 0; k < R; ++k) {                                                      \
+        T bb;                                                                                            \
+        if (do_k) bb = tb[k] - t1m[k] * v1m[k] - t1p[k] * v1p[k] - t2m[k] * v2m[k] - t2p[k] * v2p[k];    \
+        else bb = tb[k] - t1m[k] * v1m[k] - t1p[k] * v1p[k];                                             \
         const int j = (lane >> 3) + 8 * k;                                                               \
-        tl[BUF][0][rsub][j] = dl;                                                                        \
-        tl[BUF][1][rsub][j] = dd;                                                                        \
-        tl[BUF][2][rsub][j] = du;                                                                        \
+        tl[BUF][0][rsub][j] = tdl[k];                                                                    \
+        tl[BUF][1][rsub][j] = td[k];                                                                     \
+        tl[BUF][2][rsub][j] = tdu[k];                                                                    \
         tl[BUF][3][rsub][j] = bb;                                                                        \
     }
+#define MFIX_LINE_TILE(Q0, BUF) \
+    MFIX_LINE_LOAD(Q0)          \
+    MFIX_LINE_STAGE(BUF)
     const int ntiles = (n + R - 1) / R;
     if (wave == 1) {
         MFIX_LINE_TILE(0, 0)
@@ -528,6 +541,8 @@ __global__ void __launch_bounds__(128) k_line_color_sys(const Sys<T> *__restrict
         __syncthreads();
     }
 #undef MFIX_LINE_TILE
+#undef MFIX_LINE_LOAD
+#undef MFIX_LINE_STAGE
     if (wave == 0) {
         if (!info && d0 == (T)0) info = n;  // DGTSV.f:160-163
         linfo[lane] = info;
