@@ -28,7 +28,25 @@ struct DGrid {
     int cyc_x, cyc_y, cyc_z;
     int dot_all;          // dot_product_par domain: 1 = every cell (IS_SERIAL), 0 = interior
     int64_t c1, c2, n3;   // strides of I and K, DIMENSION_3
+    uint64_t mc2, mnj;    // reciprocals of c2 and nj for the cell decode (div_magic)
+    int sc2, snj;
 };
+
+// Division by an invariant divisor d >= 1 as a multiply and a shift
+// (Granlund and Montgomery, "Division by invariant integers using
+// multiplication", 1994, Thm. 4.2): with l = ceil(log2 d) and
+// m = ceil(2^(32+l) / d), floor(n / d) = (n * m) >> (32 + l) for every
+// 0 <= n < 2^32; with n < 2^31 (DIMENSION_3 < 2^31 is checked at grid set-up)
+// the product fits in 64 bits.
+inline void div_magic(uint32_t d, uint64_t &m, int &s) {
+    int l = 0;
+    while ((uint64_t(1) << l) < d) ++l;
+    s = 32 + l;
+    m = ((uint64_t(1) << s) + d - 1) / d;
+}
+__device__ __forceinline__ uint32_t div_by(uint32_t n, uint64_t m, int s) {
+    return (uint32_t)(((uint64_t)n * m) >> s);
+}
 
 // Neighbour along one axis (0-based), set_increments.f:72-112: clamped to the
 // cell itself at the array ends; wrapped for cyclic axes on interior cells.
@@ -48,13 +66,13 @@ struct Nb {
     bool interior;
 };
 
-// (i,j,k) of storage index c (0-based).  DIMENSION_3 < 2^31 (checked at grid
-// set-up), so 32-bit unsigned division suffices.
+// (i,j,k) of storage index c (0-based), by the precomputed reciprocals
+// (DIMENSION_3 < 2^31, checked at grid set-up).
 __device__ __forceinline__ void cell_ijk(const DGrid &g, int64_t c, int &i, int &j, int &k) {
     const uint32_t cc = (uint32_t)c, c2 = (uint32_t)g.c2, nj = (uint32_t)g.nj;
-    const uint32_t kk = cc / c2;
+    const uint32_t kk = div_by(cc, g.mc2, g.sc2);
     const uint32_t m = cc - kk * c2;
-    const uint32_t ii = m / nj;
+    const uint32_t ii = div_by(m, g.mnj, g.snj);
     i = (int)ii;
     j = (int)(m - ii * nj);
     k = (int)kk;

@@ -53,6 +53,8 @@ DGrid make_dgrid(const mfix_leq_grid &h) {
     g.c1 = g.nj;
     g.c2 = (int64_t)g.nj * g.ni;
     g.n3 = g.c2 * g.nk;
+    div_magic((uint32_t)g.c2, g.mc2, g.sc2);
+    div_magic((uint32_t)g.nj, g.mnj, g.snj);
     return g;
 }
 
