@@ -231,6 +231,12 @@ int mfix_leq_last_stats(mfix_leq_stats *st);
  * captured HIP graph (chunks of 16); 0: direct launches.  Results are
  * bit-identical either way. */
 int mfix_leq_set_graphs(int on);
+/* 1 (default): a mixed-precision DIAG solve whose unit-scaled diagonal is
+ * -1 in every row (1/A0 = -1 in FP32: each row's largest coefficient is its
+ * diagonal) applies the preconditioner as a negation and does not read the
+ * 1/A0 plane; 0: always read it.  Results are bit-identical either way.
+ * No counterpart in the reference (a bandwidth switch of this library). */
+int mfix_leq_set_unit_diag(int on);
 
 /* ---- timing of the dominant kernel (HIP events on the solver stream) ---- */
 /* Enable per-launch event timing of kernel `which` (0 = fused stencil

@@ -331,6 +331,11 @@ int mfix_leq_set_graphs(int on) {
     return 0;
 }
 
+int mfix_leq_set_unit_diag(int on) {
+    ctx().unit_diag = on != 0;
+    return 0;
+}
+
 int mfix_leq_last_stats(mfix_leq_stats *st) {
     if (!st) return MFIX_LEQ_EINVAL;
     *st = ctx().last_stats;

@@ -52,12 +52,13 @@ int size_class(int64_t n3);
 using Sweep = std::array<char, 4>;
 
 // Block maps of one launch sequence over the systems `ids` of a table:
-// stream kernels (smap), finish kernels (fmap), batched red-black line
+// stream kernels (smap), finish kernels (fmap), reliable updates (umap,
+// update_blocks() entries per system), batched red-black line
 // phases (lmap[pass]); LINE systems whose sweep string is not made of R/S
 // passes are swept one by one (line_seq).
 struct MapSet {
     std::vector<int> ids;
-    std::vector<int2> smap, fmap, lmap[kMaxRPasses];
+    std::vector<int2> smap, fmap, umap, lmap[kMaxRPasses];
     int nlpass = 0;
     std::vector<int> line_seq;
     double cells = 0.0;           // sum of N3 over ids (timing bytes)
@@ -65,7 +66,7 @@ struct MapSet {
     int n_line = 0, n_fused = 0;  // LINE systems / DIAG+NONE systems among ids
     DevBuf dev;
     int *d_ids = nullptr;
-    int2 *d_smap = nullptr, *d_fmap = nullptr, *d_lmap[kMaxRPasses] = {};
+    int2 *d_smap = nullptr, *d_fmap = nullptr, *d_umap = nullptr, *d_lmap[kMaxRPasses] = {};
     uint64_t version = 0;         // bumped whenever the device copy changes
 };
 
@@ -107,6 +108,7 @@ struct Context {
     Prof prof[kNumProf];
     Shape tuned[2][kSizeClasses];  // [fp32][size class]; bs == 0 means untuned
     int use_graphs = 1;            // capture chunks of iterations in HIP graphs
+    int unit_diag = 1;             // mixed DIAG: negate instead of reading a 1/A0 plane of -1s
     hipError_t last_hip_error = hipSuccess;
     char last_msg[256] = {0};
     const double *host_noise = nullptr;

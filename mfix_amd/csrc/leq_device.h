@@ -168,30 +168,42 @@ __device__ __forceinline__ float diag_apply(float r, float inv_a0) { return r * 
 // Operand fetch of the stencil: x itself, or x/A0 of the neighbour (DIAG
 // preconditioner applied on the fly, LEQ_MSOLVE1 leqsol_mod.f:729-744, so
 // Phat and Shat never exist as arrays).  Both forms round exactly like the
-// reference's separate division and matvec.
-template <typename T, bool DIV, typename PT = const T *>
+// reference's separate division and matvec.  OP_NEG is DIAG on a system
+// whose every unit-scaled A0 is exactly -1 (each row's largest coefficient
+// is its diagonal, as in MFIX's diagonally dominant equations): x/(-1) and
+// x*(1/(-1)) are exactly -x, so the A0 plane is not read at all.
+enum OperandOp : int { OP_PLAIN = 0, OP_DIAG = 1, OP_NEG = 2 };
+
+template <typename T, int OP, typename PT = const T *>
 struct Operand {
     PT x;
     PT a0;
     __device__ __forceinline__ T operator()(int64_t q) const {
-        if (DIV) return diag_apply(x[q], a0[q]);
+        if (OP == OP_DIAG) return diag_apply(x[q], a0[q]);
+        if (OP == OP_NEG) return -x[q];
         return x[q];
     }
 };
 
-// y(c) of LEQ_MATVEC in the reference's summation order.
+// y(c) of LEQ_MATVEC in the reference's summation order.  Every load of the
+// row is issued before its arithmetic, so a thread waits on memory once per
+// row: a 2-D grid (km = kp = c) still reads its K planes and the select
+// drops their terms, leaving the reference's sum untouched.
 template <typename T, typename X, typename PA>
 __device__ __forceinline__ T stencil_row(const DGrid &g, PA a, const X &x, int64_t c, const Nb &n) {
     const int64_t n3 = g.n3;
-    T v = ld_stream(&a[1 * n3 + c]) * x(n.jm) + ld_stream(&a[2 * n3 + c]) * x(n.im);
-    v = v + ld_stream(&a[3 * n3 + c]) * x(c);
-    v = v + ld_stream(&a[4 * n3 + c]) * x(n.ip);
-    v = v + ld_stream(&a[5 * n3 + c]) * x(n.jp);
-    if (g.do_k) {
-        v = v + ld_stream(&a[0 * n3 + c]) * x(n.km);
-        v = v + ld_stream(&a[6 * n3 + c]) * x(n.kp);
-    }
-    return v;
+    const T a1 = ld_stream(&a[1 * n3 + c]), a2 = ld_stream(&a[2 * n3 + c]), a3 = ld_stream(&a[3 * n3 + c]);
+    const T a4 = ld_stream(&a[4 * n3 + c]), a5 = ld_stream(&a[5 * n3 + c]);
+    const T a0 = ld_stream(&a[0 * n3 + c]), a6 = ld_stream(&a[6 * n3 + c]);
+    const T xjm = x(n.jm), xim = x(n.im), xc = x(c), xip = x(n.ip), xjp = x(n.jp);
+    const T xkm = x(n.km), xkp = x(n.kp);
+    T v = a1 * xjm + a2 * xim;
+    v = v + a3 * xc;
+    v = v + a4 * xip;
+    v = v + a5 * xjp;
+    T vk = v + a0 * xkm;
+    vk = vk + a6 * xkp;
+    return g.do_k ? vk : v;
 }
 
 // FP32 stencil in difference form: with the diagonal plane replaced by the
@@ -202,16 +214,18 @@ __device__ __forceinline__ T stencil_row(const DGrid &g, PA a, const X &x, int64
 template <typename X, typename PA>
 __device__ __forceinline__ float stencil_row_diff(const DGrid &g, PA a, const X &x, int64_t c, const Nb &n) {
     const int64_t n3 = g.n3;
-    const float xc = x(c);
-    double v = (double)(ld_stream(&a[1 * n3 + c]) * (x(n.jm) - xc)) +
-               (double)(ld_stream(&a[2 * n3 + c]) * (x(n.im) - xc));
-    v += (double)(ld_stream(&a[4 * n3 + c]) * (x(n.ip) - xc));
-    v += (double)(ld_stream(&a[5 * n3 + c]) * (x(n.jp) - xc));
-    if (g.do_k) {
-        v += (double)(ld_stream(&a[0 * n3 + c]) * (x(n.km) - xc));
-        v += (double)(ld_stream(&a[6 * n3 + c]) * (x(n.kp) - xc));
-    }
-    v += (double)ld_stream(&a[3 * n3 + c]) * (double)xc;
+    const float a1 = ld_stream(&a[1 * n3 + c]), a2 = ld_stream(&a[2 * n3 + c]), a3 = ld_stream(&a[3 * n3 + c]);
+    const float a4 = ld_stream(&a[4 * n3 + c]), a5 = ld_stream(&a[5 * n3 + c]);
+    const float a0 = ld_stream(&a[0 * n3 + c]), a6 = ld_stream(&a[6 * n3 + c]);
+    const float xc = x(c), xjm = x(n.jm), xim = x(n.im), xip = x(n.ip), xjp = x(n.jp);
+    const float xkm = x(n.km), xkp = x(n.kp);
+    double v = (double)(a1 * (xjm - xc)) + (double)(a2 * (xim - xc));
+    v += (double)(a4 * (xip - xc));
+    v += (double)(a5 * (xjp - xc));
+    double vk = v + (double)(a0 * (xkm - xc));
+    vk += (double)(a6 * (xkp - xc));
+    v = g.do_k ? vk : v;
+    v += (double)a3 * (double)xc;
     return (float)v;
 }
 
@@ -233,6 +247,15 @@ inline Part make_part(int64_t n, int bs, int cpt) {
     p.nblk = (n + p.chunk - 1) / p.chunk;
     if (p.nblk < 1) p.nblk = 1;
     return p;
+}
+
+// Blocks given to one system by the reliable-update launch: a multiple of
+// the XCD count (so every system's map range starts on XCD 0), at most
+// kUpdateBlocks.
+constexpr int kUpdateBlocks = 1024;
+__host__ __device__ __forceinline__ int64_t update_blocks(int64_t nblk) {
+    const int64_t r = (nblk + kNumXcd - 1) / kNumXcd * kNumXcd;
+    return r < kUpdateBlocks ? r : kUpdateBlocks;
 }
 
 __device__ __forceinline__ int64_t xcd_block(int64_t nblocks) {

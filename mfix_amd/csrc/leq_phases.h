@@ -33,6 +33,7 @@ struct Sys {
     int nfin;              // blocks of this system in finish launches
     int pc;                // PC_NONE / PC_DIAG / PC_LINE
     int itmax, minimize, icheck, mixed, max_updates;
+    int unit_diag;         // mixed DIAG: negate where every 1/A0 is -1 (else read the plane)
     double tol, delta;
     const T *a;            // reference layout (FP64) or difference layout (FP32)
     T *x, *r, *rt, *p, *s, *t, *v, *phat, *shat;  // x: VAR (FP64 solve only)
@@ -72,6 +73,7 @@ __global__ void k_init_states(const Sys<T> *__restrict__ sys, const int *__restr
     s.delta = S.delta;
     s.max_updates = S.max_updates;
     s.restart_always = S.mixed && S.g.n3 >= (int64_t(1) << 18);
+    s.nonunit_diag = !S.mixed || !S.unit_diag;  // the FP64 solver always divides, like the reference
     s.true_rnorm = -1.0;
     *S.st = s;
 }
@@ -89,7 +91,7 @@ __global__ void __launch_bounds__(BS) k_setup_residual(const Sys<T> *__restrict_
     double v[1] = {0.0};
     const int64_t n3 = g.n3;
     int zero = 0;
-    const Operand<double, false, gptr<const double>> X{to_global((const double *)S.xa), nullptr};
+    const Operand<double, OP_PLAIN, gptr<const double>> X{to_global((const double *)S.xa), nullptr};
     FOR_CELLS(BS, P, blk, c) {
         double amax = 0.0;
 #pragma unroll
@@ -188,17 +190,19 @@ __global__ void __launch_bounds__(BS) k_matvec_dots(const Sys<T> *__restrict__ s
     auto body = [&](auto X) {
         FOR_CELLS(BS, P, blk, c) {
             const Nb n = neighbours_fast(g, c);
+            const T wc = w[c];
             const T yy = Stencil<T>::row(g, a, X, c, n);
             y[c] = yy;
             if (all || n.interior) {
-                const double yd = (double)yy, wd = (double)w[c];
+                const double yd = (double)yy, wd = (double)wc;
                 v[0] += (MODE == 0) ? wd * yd : yd * wd;
                 if (MODE == 1) v[1] += yd * yd;
             }
         }
     };
-    if (S.pc == PC_DIAG) body(Operand<T, true, gptr<const T>>{xin, a0});
-    else body(Operand<T, false, gptr<const T>>{xin, nullptr});
+    if (S.pc == PC_DIAG && st->nonunit_diag) body(Operand<T, OP_DIAG, gptr<const T>>{xin, a0});
+    else if (S.pc == PC_DIAG) body(Operand<T, OP_NEG, gptr<const T>>{xin, nullptr});
+    else body(Operand<T, OP_PLAIN, gptr<const T>>{xin, nullptr});
     if (MODE == 0) {
         double v1[1] = {v[0]};
         block_partial<BS, 1>(v1, to_global(S.partials), P.nblk, blk, lds);
@@ -249,7 +253,8 @@ __global__ void __launch_bounds__(BS) k_xr_update(const Sys<T> *__restrict__ sys
     const double alpha = st->alpha, omega = st->omega;
     const bool sexit = st->s_exit;
     const bool yr = st->y_reset;
-    const bool div = S.pc == PC_DIAG;
+    const bool div = S.pc == PC_DIAG && st->nonunit_diag;
+    const bool neg = S.pc == PC_DIAG && !st->nonunit_diag;
     const auto ph = S.pc == PC_LINE ? to_global(S.phat) : to_global(S.p);
     const auto sh = S.pc == PC_LINE ? to_global(S.shat) : to_global(S.s);
     auto *__restrict__ a0 = to_global(S.a) + (int64_t)DiagPlane<T>::value * g.n3;
@@ -261,6 +266,7 @@ __global__ void __launch_bounds__(BS) k_xr_update(const Sys<T> *__restrict__ sys
     auto *__restrict__ r = to_global(S.r);
     auto *__restrict__ rt = to_global(S.rt);
     FOR_CELLS(BS, P, blk, c) {
+        const T rtc = rt[c];
         const double x0 = mixed ? (yr ? 0.0 : y[c]) : (double)x[c];
         double phv;
         T d0 = (T)1;
@@ -268,20 +274,20 @@ __global__ void __launch_bounds__(BS) k_xr_update(const Sys<T> *__restrict__ sys
             d0 = a0[c];
             phv = (double)diag_apply((T)ph[c], d0);
         } else {
-            phv = (double)ph[c];
+            phv = neg ? -(double)ph[c] : (double)ph[c];
         }
         if (sexit) {
             if (mixed) y[c] = x0 + alpha * phv;
             else x[c] = (T)(x0 + alpha * phv);
         } else {
-            const double shv = div ? (double)diag_apply((T)sh[c], d0) : (double)sh[c];
+            const double shv = div ? (double)diag_apply((T)sh[c], d0) : neg ? -(double)sh[c] : (double)sh[c];
             if (mixed) y[c] = x0 + alpha * phv + omega * shv;
             else x[c] = (T)(x0 + alpha * phv + omega * shv);
             const T rv = (T)((double)s[c] - omega * (double)t[c]);
             r[c] = rv;
             if (g.dot_all || is_interior(g, c)) {
                 acc[0] += (double)rv * (double)rv;
-                acc[1] += (double)rt[c] * (double)rv;
+                acc[1] += (double)rtc * (double)rv;
             }
         }
     }
@@ -293,11 +299,21 @@ __global__ void __launch_bounds__(BS) k_xr_update(const Sys<T> *__restrict__ sys
 // buffer), r = (float)r64; partial ||r64||^2, <Rtilde, r> (rho of the
 // continued recurrence) and <r, r> (rho of a restarted one, whose Rtilde is
 // r).  After a breakdown (discard_y) y is dropped: xo = xi.
+//
+// The kernel is launched every iteration but works only every ~100, so each
+// system gets at most kUpdateBlocks blocks (map entry y = 0..ub-1) that loop
+// over the logical blocks of their XCD's slab: an idle launch retires in a
+// few microseconds.  Each logical block's partials are summed exactly as a
+// one-block-per-chunk launch would sum them.
 template <int BS>
 __global__ void __launch_bounds__(BS) k_reliable_update(const Sys<float> *__restrict__ sys,
                                                         const int2 *__restrict__ map) {
     __shared__ double lds[(BS / 64) * 3];
-    SYS_BLOCK(float)
+    const int2 e_ = map[blockIdx.x];
+    if (e_.x < 0) return;
+    const Sys<float> &S = sys[e_.x];
+    const DGrid &g = S.g;
+    const Part &P = S.P;
     auto *__restrict__ st = to_global(S.st);
     if (st->done || !st->need_update) return;
     const int cur = st->x_cur;
@@ -307,25 +323,33 @@ __global__ void __launch_bounds__(BS) k_reliable_update(const Sys<float> *__rest
     auto *__restrict__ b = to_global(S.b64);
     auto *__restrict__ r = to_global(S.r);
     auto *__restrict__ rt = to_global(S.rt);
-    double acc[3] = {0.0, 0.0, 0.0};
     struct XY {
         gptr<const double> xi;
         gptr<const double> y;
         __device__ __forceinline__ double operator()(int64_t q) const { return y ? xi[q] + y[q] : xi[q]; }
     } X{xi, st->discard_y ? nullptr : to_global(S.y)};
-    FOR_CELLS(BS, P, blk, c) {
-        const Nb n = neighbours_fast(g, c);
-        const double rr = b[c] - stencil_row<double>(g, a, X, c, n);
-        xo[c] = X(c);
-        const float r32 = (float)rr;
-        r[c] = r32;
-        if (g.dot_all || n.interior) {
-            acc[0] += rr * rr;
-            acc[1] += (double)rt[c] * (double)r32;
-            acc[2] += (double)r32 * (double)r32;
+    const int64_t ub = update_blocks(P.nblk), nper = ub / kNumXcd;
+    const int64_t per = (P.nblk + kNumXcd - 1) / kNumXcd;
+    const int64_t lo = (e_.y % kNumXcd) * per, hi = lo + per < P.nblk ? lo + per : P.nblk;
+    for (int64_t blk = lo + e_.y / kNumXcd; blk < hi; blk += nper) {
+        double acc[3] = {0.0, 0.0, 0.0};
+        FOR_CELLS(BS, P, blk, c) {
+            const Nb n = neighbours_fast(g, c);
+            const float rtc = rt[c];
+            const double bc = b[c];
+            const double rr = bc - stencil_row<double>(g, a, X, c, n);
+            xo[c] = X(c);
+            const float r32 = (float)rr;
+            r[c] = r32;
+            if (g.dot_all || n.interior) {
+                acc[0] += rr * rr;
+                acc[1] += (double)rtc * (double)r32;
+                acc[2] += (double)r32 * (double)r32;
+            }
         }
+        block_partial<BS, 3>(acc, to_global(S.partials), P.nblk, blk, lds);
+        __syncthreads();  // lds is reused by the next logical block
     }
-    block_partial<BS, 3>(acc, to_global(S.partials), P.nblk, blk, lds);
 }
 
 // FP32 difference-layout copy of the (scaled) FP64 matrix: off-diagonals
@@ -340,6 +364,7 @@ __global__ void __launch_bounds__(BS) k_prepare_a32(const Sys<float> *__restrict
     auto *__restrict__ a = to_global(S.a64);
     auto *__restrict__ a32 = to_global(const_cast<float *>(S.a));
     const int64_t n3 = g.n3;
+    int nonunit = 0;
     FOR_CELLS(BS, P, blk, c) {
         double e = a[3 * n3 + c];
 #pragma unroll
@@ -352,8 +377,14 @@ __global__ void __launch_bounds__(BS) k_prepare_a32(const Sys<float> *__restrict
         a32[3 * n3 + c] = (float)e;
         // DIAG systems keep 1/A0 (the inner DIAG preconditioner multiplies),
         // LINE systems A0 (the line solves' diagonal)
-        a32[7 * n3 + c] = S.pc == PC_DIAG ? (float)(1.0 / a[3 * n3 + c]) : (float)a[3 * n3 + c];
+        const float d = S.pc == PC_DIAG ? (float)(1.0 / a[3 * n3 + c]) : (float)a[3 * n3 + c];
+        a32[7 * n3 + c] = d;
+        nonunit |= d != -1.0f;
     }
+    // a unit-scaled A0 within 2^-25 of -1 has 1/A0 = -1 in FP32, and the
+    // DIAG kernels then negate instead of reading this plane
+    if (__any(nonunit) && (threadIdx.x & 63) == 0 && S.pc == PC_DIAG)
+        atomicOr((int *)&to_global(S.st)->nonunit_diag, 1);
 }
 
 // Finish kernels over a table: finish block b serves system map[b].x as its
@@ -640,7 +671,7 @@ __global__ void __launch_bounds__(BS) k_matvec(DGrid g, Part P, const T *__restr
                                                T *__restrict__ y) {
     if (!xcd_valid(P.nblk)) return;
     const int64_t blk = xcd_block(P.nblk);
-    const Operand<T, false> X{x, nullptr};
+    const Operand<T, OP_PLAIN> X{x, nullptr};
     FOR_CELLS(BS, P, blk, c) {
         const Nb n = neighbours_fast(g, c);
         y[c] = stencil_row<T>(g, a, X, c, n);

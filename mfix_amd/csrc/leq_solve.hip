@@ -346,6 +346,7 @@ void sys_carve(Sys<T> &S, const SysDesc &d, const Part &P, char *base) {
     S.tol = d.tol;
     S.delta = d.delta;
     S.max_updates = d.max_updates;
+    S.unit_diag = ctx().unit_diag;
     S.r = (T *)take(n3 * sizeof(T));
     S.rt = (T *)take(n3 * sizeof(T));
     S.p = (T *)take(n3 * sizeof(T));
@@ -402,7 +403,7 @@ static int rb_passes(const Sweep &sw, bool do_k) {
 
 template <typename T>
 int build_maps(const Group<T> &G, const std::vector<int> &ids, MapSet &M, hipStream_t s) {
-    std::vector<int2> smap, fmap, lmap[kMaxRPasses];
+    std::vector<int2> smap, fmap, umap, lmap[kMaxRPasses];
     std::vector<int> seq;
     int nlpass = 0, n_line = 0, n_fused = 0;
     double cells = 0.0, k1 = 0.0;
@@ -410,13 +411,13 @@ int build_maps(const Group<T> &G, const std::vector<int> &ids, MapSet &M, hipStr
     for (int id : ids) {
         const Sys<T> &S = G.sys[id];
         cells += (double)S.g.n3;
-        // first stencil pass: the coefficient planes it streams (7; the FP32
-        // difference layout of a DIAG system adds its 1/A0 plane), the
-        // operand, the dot partner and the result
+        // first stencil pass: the seven coefficient planes, the operand, the
+        // dot partner and the result (an FP32 DIAG system whose 1/A0 is not
+        // uniformly -1 also reads its 1/A0 plane: not counted, so the rate
+        // derived from these bytes is a lower bound there)
         if (S.pc == PC_LINE) ++n_line;
         else ++n_fused;
-        const double planes = (S.pc == PC_DIAG && sizeof(T) == sizeof(float)) ? 8.0 : 7.0;
-        k1 += (double)S.g.n3 * (planes * sa + 3.0 * sv);
+        k1 += (double)S.g.n3 * (7.0 * sa + 3.0 * sv);
         // stream kernels: an 8-aligned range whose slot j holds the logical
         // block that puts one contiguous slab of the system on each XCD
         const int64_t nb = S.P.nblk;
@@ -429,6 +430,7 @@ int build_maps(const Group<T> &G, const std::vector<int> &ids, MapSet &M, hipStr
             smap.push_back(valid ? make_int2(id, (int)(start + r)) : make_int2(-1, -1));
         }
         for (int f = 0; f < S.nfin; ++f) fmap.push_back(make_int2(id, f));
+        for (int64_t u = 0; u < update_blocks(nb); ++u) umap.push_back(make_int2(id, (int)u));
         if (S.pc == PC_LINE) {
             const int np = rb_passes(G.sweep[id], S.g.do_k != 0);
             if (np < 0) {
@@ -446,7 +448,9 @@ int build_maps(const Group<T> &G, const std::vector<int> &ids, MapSet &M, hipStr
     bool same = M.ids == ids && M.smap.size() == smap.size() && M.fmap.size() == fmap.size() &&
                 M.nlpass == nlpass && M.line_seq == seq && M.dev.ptr;
     if (same) same = !std::memcmp(M.smap.data(), smap.data(), smap.size() * sizeof(int2)) &&
-                     !std::memcmp(M.fmap.data(), fmap.data(), fmap.size() * sizeof(int2));
+                     !std::memcmp(M.fmap.data(), fmap.data(), fmap.size() * sizeof(int2)) &&
+                     M.umap.size() == umap.size() &&
+                     !std::memcmp(M.umap.data(), umap.data(), umap.size() * sizeof(int2));
     for (int q = 0; q < kMaxRPasses && same; ++q)
         same = M.lmap[q].size() == lmap[q].size() &&
                !std::memcmp(M.lmap[q].data(), lmap[q].data(), lmap[q].size() * sizeof(int2));
@@ -458,10 +462,11 @@ int build_maps(const Group<T> &G, const std::vector<int> &ids, MapSet &M, hipStr
     M.ids = ids;
     M.smap.swap(smap);
     M.fmap.swap(fmap);
+    M.umap.swap(umap);
     for (int q = 0; q < kMaxRPasses; ++q) M.lmap[q].swap(lmap[q]);
     M.nlpass = nlpass;
     M.line_seq.swap(seq);
-    // one device block: ids | smap | fmap | lmap[0..3]
+    // one device block: ids | smap | fmap | umap | lmap[0..3]
     std::vector<char> blob;
     auto put = [&](const void *p, size_t bytes) {
         const size_t off = blob.size();
@@ -472,6 +477,7 @@ int build_maps(const Group<T> &G, const std::vector<int> &ids, MapSet &M, hipStr
     const size_t o_ids = put(M.ids.data(), M.ids.size() * sizeof(int));
     const size_t o_s = put(M.smap.data(), M.smap.size() * sizeof(int2));
     const size_t o_f = put(M.fmap.data(), M.fmap.size() * sizeof(int2));
+    const size_t o_u = put(M.umap.data(), M.umap.size() * sizeof(int2));
     size_t o_l[kMaxRPasses];
     for (int q = 0; q < kMaxRPasses; ++q) o_l[q] = put(M.lmap[q].data(), M.lmap[q].size() * sizeof(int2));
     int rc;
@@ -483,6 +489,7 @@ int build_maps(const Group<T> &G, const std::vector<int> &ids, MapSet &M, hipStr
     M.d_ids = (int *)(base + o_ids);
     M.d_smap = (int2 *)(base + o_s);
     M.d_fmap = (int2 *)(base + o_f);
+    M.d_umap = (int2 *)(base + o_u);
     for (int q = 0; q < kMaxRPasses; ++q) M.d_lmap[q] = (int2 *)(base + o_l[q]);
     M.version += 1;
     return 0;
@@ -577,7 +584,8 @@ static void iteration_bs(const Group<T> &G, const MapSet &M, hipStream_t s) {
     hipLaunchKernelGGL((k_xr_update<BS, T>), gs, blk, 0, s, G.d_sys, M.d_smap);
     if (G.mixed) {
         finish<T, FIN_XR_MIXED, 2>(G, M, s);
-        hipLaunchKernelGGL((k_reliable_update<BS>), gs, blk, 0, s, (const Sys<float> *)G.d_sys, M.d_smap);
+        hipLaunchKernelGGL((k_reliable_update<BS>), dim3((unsigned)M.umap.size()), blk, 0, s,
+                           (const Sys<float> *)G.d_sys, M.d_umap);
         finish<T, FIN_UPDATE, 3>(G, M, s);
     } else {
         finish<T, FIN_XR, 2>(G, M, s);

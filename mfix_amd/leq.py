@@ -236,6 +236,12 @@ def set_graphs(on: bool) -> None:
     check(_lib_().mfix_leq_set_graphs(int(on)), "set_graphs")
 
 
+def set_unit_diag(on: bool) -> None:
+    """Mixed DIAG solves negate instead of reading 1/A0 when every 1/A0 is -1
+    (default on); results are bit-identical either way."""
+    check(_lib_().mfix_leq_set_unit_diag(int(on)), "set_unit_diag")
+
+
 # ---- batched solves -------------------------------------------------------------
 
 @dataclass

@@ -222,3 +222,51 @@ def test_mixed_config0_iterations_close_to_fp64(gpu, oracle_mod):
     fp64 = gpu.bicgs_dev(dx2, da2, db2, pc="DIAG", tol=TOL, itmax=5000)
     assert fp64.ier == 0
     assert mixed.stats["iterations"] <= 2.0 * fp64.stats["iterations"] + 10
+
+
+def _mixed_solve(gpu, a, b, pc="DIAG", itmax=3000):
+    dx = gpu.DeviceArray.from_host(np.zeros(b.size))
+    da, db = gpu.DeviceArray.from_host(a), gpu.DeviceArray.from_host(b)
+    out = gpu.bicgs_dev(dx, da, db, pc=pc, cmethod="RSRS", tol=TOL, itmax=itmax, mixed=True)
+    return dx.download(), out
+
+
+def test_mixed_unit_diag_negation_bit_identical(gpu):
+    """Every unit-scaled A0 of a diagonally dominant system is -1 to FP32
+    precision, so mixed DIAG negates instead of reading its 1/A0 plane: the
+    solve is bit-identical to the one that reads the plane."""
+    gpu.grid_init(24, 20, 22)
+    a, b = make_system(0, 24, 20, 22, False, seed=504)
+    runs = []
+    for on in (True, False):
+        gpu.set_unit_diag(on)
+        try:
+            runs.append(_mixed_solve(gpu, a, b))
+        finally:
+            gpu.set_unit_diag(True)
+    (x1, o1), (x2, o2) = runs
+    assert o1.ier == 0 and o2.ier == 0
+    assert o1.stats["iterations"] == o2.stats["iterations"]
+    np.testing.assert_array_equal(x1, x2)
+
+
+def test_mixed_diag_with_weak_diagonal_rows(gpu, oracle_mod):
+    """Rows whose largest coefficient is an off-diagonal keep A0 != -1 after
+    unit scaling: the mixed DIAG solve reads 1/A0 and still meets TOL on the
+    FP64 true residual."""
+    im, jm, km = 16, 14, 12
+    gpu.grid_init(im, jm, km)
+    og = oracle_mod.grid(im, jm, km)
+    a, b = make_system(0, im, jm, km, False, seed=11)
+    n3 = b.size
+    nj, ni = jm + 2, im + 2
+    rows = [(1 + j) + (1 + i) * nj + (1 + k) * nj * ni for i, j, k in ((3, 4, 5), (7, 2, 9), (10, 11, 3), (5, 5, 5))]
+    for c in rows:
+        off = max(abs(a[p * n3 + c]) for p in range(7) if p != 3)
+        a[3 * n3 + c] = -0.9 * off
+    x, out = _mixed_solve(gpu, a, b)
+    assert out.ier == 0, out
+    a_s, b_s, _ = oracle_mod.unit_scale(og, a, b)
+    assert np.any(a_s[3 * n3:4 * n3] != -1.0)
+    rn0 = np.sqrt(np.dot(b_s, b_s))
+    assert _true_resid(oracle_mod, og, a_s, b_s, x) <= TOL * rn0 * 1.0000001
