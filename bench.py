@@ -326,8 +326,10 @@ def main(argv=None):
     leq.grid_init(n, n, n)
     blk = C.c_int()
     lpb = C.c_int()
+    shapes = {}
     for mixed in ((1, 0) if not args.no_tune else ()):
         _lib.check(_lib.load().mfix_leq_autotune(n, n, n, mixed, C.byref(blk), C.byref(lpb)), "autotune")
+        shapes["mixed" if mixed else "fp64"] = f"{blk.value // 100}x{blk.value % 100}"
     a0 = leq.DeviceArray.from_host(a)
     b0 = leq.DeviceArray.from_host(b)
     da, db = leq.DeviceArray(a.size), leq.DeviceArray(n3)
@@ -417,6 +419,7 @@ def main(argv=None):
             "n3": n3,
             "iterations": its,
             "reliable_updates": float(np.mean([s["refinements"] for s in stats])),
+            "launch_shape": shapes or "default",
             "latency_ms_p50": 1e3 * _percentile(times, 50),
             "latency_ms_p99": 1e3 * _percentile(times, 99),
             "fp64_solves_per_s": fp64_value,

@@ -179,6 +179,12 @@ int mfix_leq_batch_solve(int nsys, const mfix_leq_batch_item *items,
  * threads-per-block and lines-per-block. */
 int mfix_leq_autotune(int imax, int jmax, int kmax, int mixed, int *block,
                       int *lines_per_block);
+/* Pin the launch shape the autotuner would record for this grid size and
+ * precision: threads per block (256, 512 or 1024) and cells per thread
+ * (1, 2 or 4; the 16-byte grouped kernels run when it is 16/element size).
+ * No counterpart in the reference. */
+int mfix_leq_set_shape(int imax, int jmax, int kmax, int mixed, int block_threads,
+                       int cells_per_thread);
 
 /* ---- serving: continuous batching with HBM-budget admission ------------- */
 typedef struct mfix_leq_server mfix_leq_server;

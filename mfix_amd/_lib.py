@@ -102,6 +102,7 @@ _SIGS = {
     "mfix_leq_server_destroy": (C.c_int, [C.c_void_p]),
     "mfix_leq_set_graphs": (C.c_int, [C.c_int]),
     "mfix_leq_set_unit_diag": (C.c_int, [C.c_int]),
+    "mfix_leq_set_shape": (C.c_int, [C.c_int] * 6),
     "mfix_leq_dev_alloc": (C.c_int, [C.c_size_t, C.POINTER(C.c_void_p)]),
     "mfix_leq_dev_free": (C.c_int, [C.c_void_p]),
     "mfix_leq_copy_to_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),

@@ -296,6 +296,15 @@ int mfix_leq_autotune(int imax, int jmax, int kmax, int mixed, int *block, int *
     return 0;
 }
 
+int mfix_leq_set_shape(int imax, int jmax, int kmax, int mixed, int block_threads, int cells_per_thread) {
+    if (imax < 1 || jmax < 1 || kmax < 1) return MFIX_LEQ_EINVAL;
+    if (block_threads != 256 && block_threads != 512 && block_threads != 1024) return MFIX_LEQ_EINVAL;
+    if (cells_per_thread != 1 && cells_per_thread != 2 && cells_per_thread != 4) return MFIX_LEQ_EINVAL;
+    const int64_t n3 = (int64_t)(imax + 2) * (jmax + 2) * (kmax + 2);
+    ctx().tuned[mixed ? 1 : 0][size_class(n3)] = Shape{block_threads, cells_per_thread, 64};
+    return 0;
+}
+
 int mfix_leq_prof_enable(int which, int on) {
     if (which < 0 || which >= kNumProf) return MFIX_LEQ_EINVAL;
     Prof &p = ctx().prof[which];

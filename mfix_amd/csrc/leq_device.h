@@ -165,6 +165,39 @@ __device__ __forceinline__ auto ld_stream(P p) {
 __device__ __forceinline__ double diag_apply(double r, double a0) { return r / a0; }
 __device__ __forceinline__ float diag_apply(float r, float inv_a0) { return r * inv_a0; }
 
+// ---- 16-byte cell groups -----------------------------------------------------
+//
+// The fused streaming kernels give each thread VN = 16/sizeof(T) consecutive
+// cells (4 FP32, 2 FP64) and move each array with one 16-byte access per
+// lane: a quarter of the load instructions and of the wave launches of one
+// cell per thread.  The vectors need only element alignment (neighbour
+// groups at c +- c1 and plane starts p*n3 are generally not 16-byte
+// aligned); gfx950 serves such accesses as dwordx4.
+typedef float f4v __attribute__((ext_vector_type(4), aligned(4)));
+typedef double d2v __attribute__((ext_vector_type(2), aligned(8)));
+typedef double d4v __attribute__((ext_vector_type(4), aligned(8)));
+
+template <typename E, int N> struct VecOf;
+template <> struct VecOf<float, 4> { using type = f4v; };
+template <> struct VecOf<double, 2> { using type = d2v; };
+template <> struct VecOf<double, 4> { using type = d4v; };
+
+template <typename T> struct Group16 { static constexpr int N = 16 / sizeof(T); };
+template <typename E, int N> using vec_t = typename VecOf<E, N>::type;
+
+template <typename E, int N, typename P>
+__device__ __forceinline__ vec_t<E, N> ldv(P p) {
+    return *(const __attribute__((address_space(1))) vec_t<E, N> *)p;
+}
+template <typename E, int N, typename P>
+__device__ __forceinline__ vec_t<E, N> ldv_nt(P p) {
+    return __builtin_nontemporal_load((const __attribute__((address_space(1))) vec_t<E, N> *)p);
+}
+template <typename E, int N, typename P>
+__device__ __forceinline__ void stv(P p, vec_t<E, N> v) {
+    *(__attribute__((address_space(1))) vec_t<E, N> *)p = v;
+}
+
 // Operand fetch of the stencil: x itself, or x/A0 of the neighbour (DIAG
 // preconditioner applied on the fly, LEQ_MSOLVE1 leqsol_mod.f:729-744, so
 // Phat and Shat never exist as arrays).  Both forms round exactly like the
@@ -185,10 +218,42 @@ struct Operand {
     }
 };
 
-// y(c) of LEQ_MATVEC in the reference's summation order.  Every load of the
-// row is issued before its arithmetic, so a thread waits on memory once per
-// row: a 2-D grid (km = kp = c) still reads its K planes and the select
+// Row arithmetic of LEQ_MATVEC in the reference's summation order, on
+// values already loaded (a_l = plane l+3 of the row, x.. = operand values).
+// A 2-D grid (km = kp = c) passes its K-plane values too and the select
 // drops their terms, leaving the reference's sum untouched.
+template <typename T>
+__device__ __forceinline__ T stencil_combine(bool do_k, T a0, T a1, T a2, T a3, T a4, T a5, T a6, T xkm, T xim,
+                                             T xjm, T xc, T xjp, T xip, T xkp) {
+    T v = a1 * xjm + a2 * xim;
+    v = v + a3 * xc;
+    v = v + a4 * xip;
+    v = v + a5 * xjp;
+    T vk = v + a0 * xkm;
+    vk = vk + a6 * xkp;
+    return do_k ? vk : v;
+}
+
+// FP32 stencil in difference form: with the diagonal plane replaced by the
+// row excess e = A0 + sum(off-diagonals) (formed in FP64, then rounded),
+// y = sum_l a_l (x_nb - x_c) + e x_c is algebraically LEQ_MATVEC but avoids
+// the cancellation between A0 x_c and the neighbour terms that costs a
+// diffusion operator's smooth modes most of FP32's digits.  a3 = e here.
+__device__ __forceinline__ float stencil_combine_diff(bool do_k, float a0, float a1, float a2, float a3, float a4,
+                                                      float a5, float a6, float xkm, float xim, float xjm, float xc,
+                                                      float xjp, float xip, float xkp) {
+    double v = (double)(a1 * (xjm - xc)) + (double)(a2 * (xim - xc));
+    v += (double)(a4 * (xip - xc));
+    v += (double)(a5 * (xjp - xc));
+    double vk = v + (double)(a0 * (xkm - xc));
+    vk += (double)(a6 * (xkp - xc));
+    v = do_k ? vk : v;
+    v += (double)a3 * (double)xc;
+    return (float)v;
+}
+
+// y(c) of LEQ_MATVEC.  Every load of the row is issued before its
+// arithmetic, so a thread waits on memory once per row.
 template <typename T, typename X, typename PA>
 __device__ __forceinline__ T stencil_row(const DGrid &g, PA a, const X &x, int64_t c, const Nb &n) {
     const int64_t n3 = g.n3;
@@ -197,20 +262,9 @@ __device__ __forceinline__ T stencil_row(const DGrid &g, PA a, const X &x, int64
     const T a0 = ld_stream(&a[0 * n3 + c]), a6 = ld_stream(&a[6 * n3 + c]);
     const T xjm = x(n.jm), xim = x(n.im), xc = x(c), xip = x(n.ip), xjp = x(n.jp);
     const T xkm = x(n.km), xkp = x(n.kp);
-    T v = a1 * xjm + a2 * xim;
-    v = v + a3 * xc;
-    v = v + a4 * xip;
-    v = v + a5 * xjp;
-    T vk = v + a0 * xkm;
-    vk = vk + a6 * xkp;
-    return g.do_k ? vk : v;
+    return stencil_combine<T>(g.do_k, a0, a1, a2, a3, a4, a5, a6, xkm, xim, xjm, xc, xjp, xip, xkp);
 }
 
-// FP32 stencil in difference form: with the diagonal plane replaced by the
-// row excess e = A0 + sum(off-diagonals) (formed in FP64, then rounded),
-// y = sum_l a_l (x_nb - x_c) + e x_c is algebraically LEQ_MATVEC but avoids
-// the cancellation between A0 x_c and the neighbour terms that costs a
-// diffusion operator's smooth modes most of FP32's digits.
 template <typename X, typename PA>
 __device__ __forceinline__ float stencil_row_diff(const DGrid &g, PA a, const X &x, int64_t c, const Nb &n) {
     const int64_t n3 = g.n3;
@@ -219,14 +273,7 @@ __device__ __forceinline__ float stencil_row_diff(const DGrid &g, PA a, const X 
     const float a0 = ld_stream(&a[0 * n3 + c]), a6 = ld_stream(&a[6 * n3 + c]);
     const float xc = x(c), xjm = x(n.jm), xim = x(n.im), xip = x(n.ip), xjp = x(n.jp);
     const float xkm = x(n.km), xkp = x(n.kp);
-    double v = (double)(a1 * (xjm - xc)) + (double)(a2 * (xim - xc));
-    v += (double)(a4 * (xip - xc));
-    v += (double)(a5 * (xjp - xc));
-    double vk = v + (double)(a0 * (xkm - xc));
-    vk += (double)(a6 * (xkp - xc));
-    v = g.do_k ? vk : v;
-    v += (double)a3 * (double)xc;
-    return (float)v;
+    return stencil_combine_diff(g.do_k, a0, a1, a2, a3, a4, a5, a6, xkm, xim, xjm, xc, xjp, xip, xkp);
 }
 
 // ---- launch shape ---------------------------------------------------------

@@ -72,11 +72,19 @@ template <> struct Stencil<double> {
     __device__ __forceinline__ static double row(const DGrid &g, PA a, const X &x, int64_t c, const Nb &n) {
         return stencil_row<double>(g, a, x, c, n);
     }
+    template <typename... V>
+    __device__ __forceinline__ static double combine(bool do_k, V... v) {
+        return stencil_combine<double>(do_k, v...);
+    }
 };
 template <> struct Stencil<float> {
     template <typename X, typename PA>
     __device__ __forceinline__ static float row(const DGrid &g, PA a, const X &x, int64_t c, const Nb &n) {
         return stencil_row_diff(g, a, x, c, n);
+    }
+    template <typename... V>
+    __device__ __forceinline__ static float combine(bool do_k, V... v) {
+        return stencil_combine_diff(do_k, v...);
     }
 };
 

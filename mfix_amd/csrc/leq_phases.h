@@ -141,12 +141,17 @@ __global__ void __launch_bounds__(BS) k_shadow(const Sys<T> *__restrict__ sys, c
     block_partial<BS, 1>(v, to_global(S.partials), P.nblk, blk, lds);
 }
 
+// The three update kernels below run BS/VN threads per logical block of BS
+// cells, each thread one 16-byte group of VN cells (VN = 4 FP32, 2 FP64);
+// a logical block of another size falls back to one cell per step.
+//
 // P = R (i=1, or after a restart) or P = R + beta*(P - omega*V)
 // (leq_bicgs.f:334-343).  A restart also resets Rtilde = R.  For LINE the
 // preconditioner's start value Phat = P (z = r, leqsol_mod.f:341-357) is
 // written in the same pass.
 template <int BS, typename T>
-__global__ void __launch_bounds__(BS) k_p_update(const Sys<T> *__restrict__ sys, const int2 *__restrict__ map) {
+__global__ void __launch_bounds__(BS / Group16<T>::N) k_p_update(const Sys<T> *__restrict__ sys,
+                                                                 const int2 *__restrict__ map) {
     SYS_BLOCK(T)
     auto *__restrict__ st = to_global(S.st);
     if (st->done) return;
@@ -156,16 +161,38 @@ __global__ void __launch_bounds__(BS) k_p_update(const Sys<T> *__restrict__ sys,
     auto *__restrict__ r = to_global(S.r);
     auto *__restrict__ p = to_global(S.p);
     auto *__restrict__ v = to_global(S.v);
+    auto *__restrict__ rt = to_global(S.rt);
     const auto ph = S.pc == PC_LINE ? to_global(S.phat) : nullptr;
-    FOR_CELLS(BS, P, blk, c) {
+    auto cell = [&](int64_t c) {
         const T rc = r[c];
-        if (restart) to_global(S.rt)[c] = rc;
+        if (restart) rt[c] = rc;
         double pv;
         if (first) pv = (double)rc;
         else pv = (double)rc + beta * ((double)p[c] - omega * (double)v[c]);
         p[c] = (T)pv;
         if (ph) ph[c] = (T)pv;
+    };
+    constexpr int VN = Group16<T>::N, TB = BS / VN;
+    if (P.chunk == (int64_t)BS) {
+        const int64_t c0 = blk * P.chunk + (int64_t)VN * threadIdx.x;
+        if (c0 + VN <= P.n) {
+            const vec_t<T, VN> rv = ldv<T, VN>(r + c0);
+            if (restart) stv<T, VN>(rt + c0, rv);
+            vec_t<T, VN> pv = rv;
+            if (!first) {
+                const vec_t<T, VN> po = ldv<T, VN>(p + c0), vo = ldv<T, VN>(v + c0);
+#pragma unroll
+                for (int l = 0; l < VN; ++l)
+                    pv[l] = (T)((double)rv[l] + beta * ((double)po[l] - omega * (double)vo[l]));
+            }
+            stv<T, VN>(p + c0, pv);
+            if (ph) stv<T, VN>(ph + c0, pv);
+        } else {
+            for (int64_t c = c0; c < P.n; ++c) cell(c);
+        }
+        return;
     }
+    FOR_CELLS(TB, P, blk, c) cell(c);
 }
 
 // MODE 0: V = A*Phat (leq_bicgs.f:355) and <Rtilde,V> (:357).
@@ -173,6 +200,9 @@ __global__ void __launch_bounds__(BS) k_p_update(const Sys<T> *__restrict__ sys,
 // The operand is P/A0 or S/A0 for DIAG, the preconditioned array for LINE,
 // and P or S for NONE.  The S-norm exit (:372-376) is decided with omega,
 // after this pass: T is then unused, so the iterates do not change.
+// One cell per thread (measured faster here than 16-byte cell groups: the
+// operand groups at c +- c1 straddle the 16-byte grid, and a group that
+// crosses the end of a J row serializes its cells, stalling its wave).
 template <int BS, typename T, int MODE>
 __global__ void __launch_bounds__(BS) k_matvec_dots(const Sys<T> *__restrict__ sys, const int2 *__restrict__ map) {
     __shared__ double lds[(BS / 64) * 2];
@@ -188,7 +218,7 @@ __global__ void __launch_bounds__(BS) k_matvec_dots(const Sys<T> *__restrict__ s
     auto *__restrict__ w = MODE == 0 ? to_global(S.rt) : to_global(S.s);
     const auto xin = MODE == 0 ? (S.pc == PC_LINE ? to_global(S.phat) : to_global(S.p)) : (S.pc == PC_LINE ? to_global(S.shat) : to_global(S.s));
     auto body = [&](auto X) {
-        FOR_CELLS(BS, P, blk, c) {
+        auto cell = [&](int64_t c) {
             const Nb n = neighbours_fast(g, c);
             const T wc = w[c];
             const T yy = Stencil<T>::row(g, a, X, c, n);
@@ -198,7 +228,8 @@ __global__ void __launch_bounds__(BS) k_matvec_dots(const Sys<T> *__restrict__ s
                 v[0] += (MODE == 0) ? wd * yd : yd * wd;
                 if (MODE == 1) v[1] += yd * yd;
             }
-        }
+        };
+        FOR_CELLS(BS, P, blk, c) cell(c);
     };
     if (S.pc == PC_DIAG && st->nonunit_diag) body(Operand<T, OP_DIAG, gptr<const T>>{xin, a0});
     else if (S.pc == PC_DIAG) body(Operand<T, OP_NEG, gptr<const T>>{xin, nullptr});
@@ -215,8 +246,10 @@ __global__ void __launch_bounds__(BS) k_matvec_dots(const Sys<T> *__restrict__ s
 // the preconditioner's start value Shat = S.  The S-norm exit test is applied
 // by the finish after the second stencil pass.
 template <int BS, typename T>
-__global__ void __launch_bounds__(BS) k_s_update(const Sys<T> *__restrict__ sys, const int2 *__restrict__ map) {
-    __shared__ double lds[(BS / 64) * 1];
+__global__ void __launch_bounds__(BS / Group16<T>::N) k_s_update(const Sys<T> *__restrict__ sys,
+                                                                 const int2 *__restrict__ map) {
+    constexpr int VN = Group16<T>::N, TB = BS / VN;
+    __shared__ double lds[(TB / 64) * 1];
     SYS_BLOCK(T)
     auto *__restrict__ st = to_global(S.st);
     if (st->done) return;
@@ -226,15 +259,34 @@ __global__ void __launch_bounds__(BS) k_s_update(const Sys<T> *__restrict__ sys,
     auto *__restrict__ v = to_global(S.v);
     auto *__restrict__ s = to_global(S.s);
     const auto sh = S.pc == PC_LINE ? to_global(S.shat) : nullptr;
-    FOR_CELLS(BS, P, blk, c) {
+    auto cell = [&](int64_t c) {
         const T sv = (T)((double)r[c] - alpha * (double)v[c]);
         s[c] = sv;
         if (sh) sh[c] = sv;
         if (g.dot_all || is_interior(g, c)) acc[0] += (double)sv * (double)sv;
+    };
+
+    if (P.chunk == (int64_t)BS) {
+        const int64_t c0 = blk * P.chunk + (int64_t)VN * threadIdx.x;
+        if (c0 + VN <= P.n) {
+            const vec_t<T, VN> rv = ldv<T, VN>(r + c0), vv = ldv<T, VN>(v + c0);
+            vec_t<T, VN> sv;
+#pragma unroll
+            for (int l = 0; l < VN; ++l) sv[l] = (T)((double)rv[l] - alpha * (double)vv[l]);
+            stv<T, VN>(s + c0, sv);
+            if (sh) stv<T, VN>(sh + c0, sv);
+#pragma unroll
+            for (int l = 0; l < VN; ++l)
+                if (g.dot_all || is_interior(g, c0 + l)) acc[0] += (double)(T)sv[l] * (double)(T)sv[l];
+        } else {
+            for (int64_t c = c0; c < P.n; ++c) cell(c);
+        }
+    } else {
+        FOR_CELLS(TB, P, blk, c) cell(c);
     }
     // slot 2: the second stencil pass writes slots 0 and 1, and one finish
     // launch after it takes all three
-    block_partial<BS, 1>(acc, to_global(S.partials) + 2 * P.nblk, P.nblk, blk, lds);
+    block_partial<TB, 1>(acc, to_global(S.partials) + 2 * P.nblk, P.nblk, blk, lds);
 }
 
 // Var += alpha*Phat + omega*Shat; R = S - omega*T (leq_bicgs.f:432-436);
@@ -244,8 +296,10 @@ __global__ void __launch_bounds__(BS) k_s_update(const Sys<T> *__restrict__ sys,
 // update) with the same FP32-rounded Phat/Shat the stencil used: keeping y
 // in FP64 stops its rounding from polluting every reliable update.
 template <int BS, typename T>
-__global__ void __launch_bounds__(BS) k_xr_update(const Sys<T> *__restrict__ sys, const int2 *__restrict__ map) {
-    __shared__ double lds[(BS / 64) * 2];
+__global__ void __launch_bounds__(BS / Group16<T>::N) k_xr_update(const Sys<T> *__restrict__ sys,
+                                                                  const int2 *__restrict__ map) {
+    constexpr int VN = Group16<T>::N, TB = BS / VN;
+    __shared__ double lds[(TB / 64) * 2];
     SYS_BLOCK(T)
     auto *__restrict__ st = to_global(S.st);
     if (st->done) return;
@@ -265,22 +319,18 @@ __global__ void __launch_bounds__(BS) k_xr_update(const Sys<T> *__restrict__ sys
     auto *__restrict__ t = to_global(S.t);
     auto *__restrict__ r = to_global(S.r);
     auto *__restrict__ rt = to_global(S.rt);
-    FOR_CELLS(BS, P, blk, c) {
+    // preconditioned Phat/Shat of one cell, rounded as the stencil used them
+    auto pre = [&](T z, T d0) -> double { return div ? (double)diag_apply(z, d0) : neg ? -(double)z : (double)z; };
+    auto cell = [&](int64_t c) {
         const T rtc = rt[c];
         const double x0 = mixed ? (yr ? 0.0 : y[c]) : (double)x[c];
-        double phv;
-        T d0 = (T)1;
-        if (div) {
-            d0 = a0[c];
-            phv = (double)diag_apply((T)ph[c], d0);
-        } else {
-            phv = neg ? -(double)ph[c] : (double)ph[c];
-        }
+        const T d0 = div ? (T)a0[c] : (T)1;
+        const double phv = pre((T)ph[c], d0);
         if (sexit) {
             if (mixed) y[c] = x0 + alpha * phv;
             else x[c] = (T)(x0 + alpha * phv);
         } else {
-            const double shv = div ? (double)diag_apply((T)sh[c], d0) : neg ? -(double)sh[c] : (double)sh[c];
+            const double shv = pre((T)sh[c], d0);
             if (mixed) y[c] = x0 + alpha * phv + omega * shv;
             else x[c] = (T)(x0 + alpha * phv + omega * shv);
             const T rv = (T)((double)s[c] - omega * (double)t[c]);
@@ -290,8 +340,55 @@ __global__ void __launch_bounds__(BS) k_xr_update(const Sys<T> *__restrict__ sys
                 acc[1] += (double)rtc * (double)rv;
             }
         }
+    };
+
+    // (an FP32 table is always the inner solve of the mixed path)
+    if (P.chunk == (int64_t)BS && (mixed || sizeof(T) == sizeof(double))) {
+        const int64_t c0 = blk * P.chunk + (int64_t)VN * threadIdx.x;
+        if (c0 + VN <= P.n) {
+            const vec_t<T, VN> PH = ldv<T, VN>(ph + c0);
+            vec_t<T, VN> D0 = (T)1;
+            if (div) D0 = ldv<T, VN>(a0 + c0);
+            vec_t<double, VN> X0 = 0.0;
+            if (mixed) {
+                if (!yr) X0 = ldv<double, VN>(y + c0);
+            } else if constexpr (sizeof(T) == sizeof(double)) {
+                X0 = ldv<double, VN>(x + c0);
+            }
+            vec_t<double, VN> XN;
+            if (sexit) {
+#pragma unroll
+                for (int l = 0; l < VN; ++l) XN[l] = X0[l] + alpha * pre(PH[l], div ? (T)D0[l] : (T)1);
+            } else {
+                const vec_t<T, VN> SH = ldv<T, VN>(sh + c0), SV = ldv<T, VN>(s + c0), TV = ldv<T, VN>(t + c0);
+                const vec_t<T, VN> RT = ldv<T, VN>(rt + c0);
+                vec_t<T, VN> RV;
+#pragma unroll
+                for (int l = 0; l < VN; ++l) {
+                    const T d0 = div ? (T)D0[l] : (T)1;
+                    XN[l] = X0[l] + alpha * pre(PH[l], d0) + omega * pre(SH[l], d0);
+                    RV[l] = (T)((double)SV[l] - omega * (double)TV[l]);
+                }
+                stv<T, VN>(r + c0, RV);
+#pragma unroll
+                for (int l = 0; l < VN; ++l)
+                    if (g.dot_all || is_interior(g, c0 + l)) {
+                        acc[0] += (double)(T)RV[l] * (double)(T)RV[l];
+                        acc[1] += (double)(T)RT[l] * (double)(T)RV[l];
+                    }
+            }
+            if (mixed) {
+                stv<double, VN>(y + c0, XN);
+            } else if constexpr (sizeof(T) == sizeof(double)) {
+                stv<double, VN>(x + c0, XN);
+            }
+        } else {
+            for (int64_t c = c0; c < P.n; ++c) cell(c);
+        }
+    } else {
+        FOR_CELLS(TB, P, blk, c) cell(c);
     }
-    block_partial<BS, 2>(acc, to_global(S.partials), P.nblk, blk, lds);
+    block_partial<TB, 2>(acc, to_global(S.partials), P.nblk, blk, lds);
 }
 
 // Reliable update (mixed path): xo = xi + y, r64 = b - A64*(xi + y) with the

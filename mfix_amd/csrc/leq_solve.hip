@@ -571,17 +571,18 @@ static void iteration_bs(const Group<T> &G, const MapSet &M, hipStream_t s) {
     const dim3 gs((unsigned)M.smap.size()), blk(BS);
     Prof &pf = ctx().prof[0];
     const bool line = M.n_line > 0;
-    hipLaunchKernelGGL((k_p_update<BS, T>), gs, blk, 0, s, G.d_sys, M.d_smap);
+    const dim3 gblk(BS / Group16<T>::N);  // grouped update kernels: 16 bytes per lane and array
+    hipLaunchKernelGGL((k_p_update<BS, T>), gs, gblk, 0, s, G.d_sys, M.d_smap);
     if (line) line_sweeps(G, M, 0, s);
     pf.begin(s);
     hipLaunchKernelGGL((k_matvec_dots<BS, T, 0>), gs, blk, 0, s, G.d_sys, M.d_smap);
     pf.end(s, M.k1_bytes);
     finish<T, FIN_ALPHA, 1>(G, M, s);
-    hipLaunchKernelGGL((k_s_update<BS, T>), gs, blk, 0, s, G.d_sys, M.d_smap);
+    hipLaunchKernelGGL((k_s_update<BS, T>), gs, gblk, 0, s, G.d_sys, M.d_smap);
     if (line) line_sweeps(G, M, 1, s);
     hipLaunchKernelGGL((k_matvec_dots<BS, T, 1>), gs, blk, 0, s, G.d_sys, M.d_smap);
     finish<T, FIN_OMEGA, 3>(G, M, s);
-    hipLaunchKernelGGL((k_xr_update<BS, T>), gs, blk, 0, s, G.d_sys, M.d_smap);
+    hipLaunchKernelGGL((k_xr_update<BS, T>), gs, gblk, 0, s, G.d_sys, M.d_smap);
     if (G.mixed) {
         finish<T, FIN_XR_MIXED, 2>(G, M, s);
         hipLaunchKernelGGL((k_reliable_update<BS>), dim3((unsigned)M.umap.size()), blk, 0, s,
@@ -1046,7 +1047,9 @@ int autotune(int imax, int jmax, int kmax, int mixed, Shape *best_out) {
     hipStream_t s = C.stream;
     (void)hipMemcpyAsync(da0, ha.data(), sizeof(double) * 7 * n3, hipMemcpyHostToDevice, s);
     (void)hipMemcpyAsync(db, hb.data(), sizeof(double) * n3, hipMemcpyHostToDevice, s);
-    const Shape cands[] = {{256, 1, 64}, {256, 2, 64}, {512, 1, 64}, {512, 2, 64}, {1024, 1, 64}, {256, 4, 64}};
+    // one logical block = one block of the stencil passes (a thread per
+    // cell); the grouped update kernels cover it with 16-byte cell groups
+    const Shape cands[] = {{256, 1, 64}, {512, 1, 64}, {1024, 1, 64}};
     const int cls = size_class(n3);
     Shape best = cands[0];
     double best_ms = 1e30;
