@@ -186,6 +186,51 @@ int mfix_leq_autotune(int imax, int jmax, int kmax, int mixed, int *block,
 int mfix_leq_set_shape(int imax, int jmax, int kmax, int mixed, int block_threads,
                        int cells_per_thread);
 
+/* ---- residual norms of an assembled equation ---------------------------- */
+/* CALC_RESID_C / CALC_RESID_S / CALC_RESID_PP (model/calc_resid.f:16-178,
+ * :195-371, :391-534), serial semantics.  var/a_m/b_m: the phase's VAR,
+ * A_M(:,:,M), B_M(:,M) in the storage contract above; flag: MFIX's FLAG
+ * array (INTEGER, DIMENSION_3; 1 = fluid cell, >= 100 walls).  Cells of the
+ * first interior layers (IS_ON_myPE_wobnd) that are fluid are summed; for C
+ * a wall neighbour stands for the cell itself (EAST_OF etc.,
+ * model/set_index1a.f:85-222).  debug_resid = DEBUG_RESID
+ * (model/run_mod.f:244): with 0, C and S return num/den/ncells only (resid
+ * and ijk_resid 0, max_resid -1; the reference leaves RESID and IJK_RESID
+ * unset) and PP returns RESID with the unnormalised maximum.  ijk_resid is
+ * 1-based like the reference's IJK. */
+typedef struct mfix_leq_resid {
+    double num, den;             /* NUM, DEN */
+    double resid;                /* RESID */
+    double max_resid;            /* MAX_RESID */
+    int ijk_resid;               /* IJK_RESID */
+    int ncells;                  /* fluid cells counted (NCELLS) */
+} mfix_leq_resid;
+
+/* CALC_RESID_C(VAR, A_M, B_M, M, NUM, DEN, RESID, MAX_RESID, IJK_RESID),
+ * model/calc_resid.f:16: host arrays (copied to the device) and device
+ * pointers (stream may be 0). */
+int mfix_calc_resid_c(const double *var, const double *a_m, const double *b_m,
+                      const int *flag, int debug_resid, mfix_leq_resid *out);
+int mfix_calc_resid_c_dev(const double *d_var, const double *d_a_m,
+                          const double *d_b_m, const int *d_flag,
+                          int debug_resid, mfix_leq_resid *out, void *stream);
+/* CALC_RESID_S(..., TOL), model/calc_resid.f:195: cells with |VAR| <= tol
+ * are skipped, and for a solids phase (M > 0) those with EP_s(:,M) <=
+ * dil_ep_s (DIL_EP_s = 1e-4, model/toleranc_mod.f:24); ep_s = NULL for M = 0. */
+int mfix_calc_resid_s(const double *var, const double *a_m, const double *b_m,
+                      const int *flag, double tol, const double *ep_s,
+                      double dil_ep_s, int debug_resid, mfix_leq_resid *out);
+int mfix_calc_resid_s_dev(const double *d_var, const double *d_a_m,
+                          const double *d_b_m, const int *d_flag, double tol,
+                          const double *d_ep_s, double dil_ep_s,
+                          int debug_resid, mfix_leq_resid *out, void *stream);
+/* CALC_RESID_PP(B_M, NORM, NUM, DEN, RESID, MAX_RESID, IJK_RESID),
+ * model/calc_resid.f:391: |B_M| of the pressure-correction equation. */
+int mfix_calc_resid_pp(const double *b_m, const int *flag, double norm,
+                       int debug_resid, mfix_leq_resid *out);
+int mfix_calc_resid_pp_dev(const double *d_b_m, const int *d_flag, double norm,
+                           int debug_resid, mfix_leq_resid *out, void *stream);
+
 /* ---- serving: continuous batching with HBM-budget admission ------------- */
 typedef struct mfix_leq_server mfix_leq_server;
 typedef struct mfix_leq_request {

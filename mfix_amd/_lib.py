@@ -43,6 +43,16 @@ class Opts(C.Structure):
     ]
 
 
+class Resid(C.Structure):
+    _fields_ = [
+        ("num", C.c_double), ("den", C.c_double), ("resid", C.c_double), ("max_resid", C.c_double),
+        ("ijk_resid", C.c_int), ("ncells", C.c_int),
+    ]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class BatchItem(C.Structure):
     _fields_ = [
         ("imax", C.c_int), ("jmax", C.c_int), ("kmax", C.c_int),
@@ -103,6 +113,15 @@ _SIGS = {
     "mfix_leq_set_graphs": (C.c_int, [C.c_int]),
     "mfix_leq_set_unit_diag": (C.c_int, [C.c_int]),
     "mfix_leq_set_shape": (C.c_int, [C.c_int] * 6),
+    "mfix_calc_resid_c": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
+    "mfix_calc_resid_c_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                        C.c_void_p]),
+    "mfix_calc_resid_s": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p,
+                                    C.c_double, C.c_int, C.c_void_p]),
+    "mfix_calc_resid_s_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p,
+                                        C.c_double, C.c_int, C.c_void_p, C.c_void_p]),
+    "mfix_calc_resid_pp": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.c_void_p]),
+    "mfix_calc_resid_pp_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.c_void_p, C.c_void_p]),
     "mfix_leq_dev_alloc": (C.c_int, [C.c_size_t, C.POINTER(C.c_void_p)]),
     "mfix_leq_dev_free": (C.c_int, [C.c_void_p]),
     "mfix_leq_copy_to_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),

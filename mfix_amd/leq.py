@@ -19,7 +19,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
-from ._lib import BatchItem, Grid, Opts, Stats, check
+from ._lib import BatchItem, Grid, Opts, Resid, Stats, check
 
 
 class SolverAbort(RuntimeError):
@@ -287,3 +287,54 @@ def batch_solve(systems: list[BatchSystem]) -> list[SolveResult]:
         raise SolverAbort("preconditioner option not found - check mfix.dat and readme")
     check(rc, "mfix_leq_batch_solve")
     return [SolveResult(stats[q].ier, stats[q].as_dict()) for q in range(n)]
+
+
+# ---- residual norms: CALC_RESID_C / _S / _PP (model/calc_resid.f) ----------
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def calc_resid(kind: str, *, b, flag, var=None, a=None, tol: float = 0.0, ep_s=None, dil_ep_s: float = 1.0e-4,
+               norm: float = 1.0, debug: bool = True) -> dict:
+    """CALC_RESID_C ("C"), CALC_RESID_S ("S") or CALC_RESID_PP ("PP") of
+    host arrays (model/calc_resid.f:16, :195, :391): var = VAR, a = A_M of
+    the phase, b = B_M of the phase, flag = MFIX's FLAG (int32), ep_s = the
+    solids phase's EP_s (S with M > 0) or None.  Returns NUM, DEN, RESID,
+    MAX_RESID, IJK_RESID (1-based) and the fluid-cell count."""
+    out = Resid()
+    fl = np.ascontiguousarray(flag, dtype=np.int32)
+    L = _lib_()
+    if kind == "PP":
+        check(L.mfix_calc_resid_pp(_f64(b).ctypes.data, fl.ctypes.data, norm, int(debug), C.byref(out)),
+              "mfix_calc_resid_pp")
+    elif kind == "C":
+        check(L.mfix_calc_resid_c(_f64(var).ctypes.data, _f64(a).ctypes.data, _f64(b).ctypes.data, fl.ctypes.data,
+                                  int(debug), C.byref(out)), "mfix_calc_resid_c")
+    elif kind == "S":
+        eps = None if ep_s is None else _f64(ep_s)
+        check(L.mfix_calc_resid_s(_f64(var).ctypes.data, _f64(a).ctypes.data, _f64(b).ctypes.data, fl.ctypes.data,
+                                  tol, None if eps is None else eps.ctypes.data, dil_ep_s, int(debug), C.byref(out)),
+              "mfix_calc_resid_s")
+    else:
+        raise ValueError(f"unknown residual kind {kind!r}")
+    return out.as_dict()
+
+
+def calc_resid_dev(kind: str, *, b: "DeviceArray", flag: "DeviceArray", var: "DeviceArray | None" = None,
+                   a: "DeviceArray | None" = None, tol: float = 0.0, ep_s: "DeviceArray | None" = None,
+                   dil_ep_s: float = 1.0e-4, norm: float = 1.0, debug: bool = True) -> dict:
+    """calc_resid on device-resident arrays (flag: an int32 DeviceArray)."""
+    out = Resid()
+    L = _lib_()
+    if kind == "PP":
+        check(L.mfix_calc_resid_pp_dev(b.ptr, flag.ptr, norm, int(debug), C.byref(out), None), "mfix_calc_resid_pp_dev")
+    elif kind == "C":
+        check(L.mfix_calc_resid_c_dev(var.ptr, a.ptr, b.ptr, flag.ptr, int(debug), C.byref(out), None),
+              "mfix_calc_resid_c_dev")
+    elif kind == "S":
+        check(L.mfix_calc_resid_s_dev(var.ptr, a.ptr, b.ptr, flag.ptr, tol, None if ep_s is None else ep_s.ptr,
+                                      dil_ep_s, int(debug), C.byref(out), None), "mfix_calc_resid_s_dev")
+    else:
+        raise ValueError(f"unknown residual kind {kind!r}")
+    return out.as_dict()

@@ -40,6 +40,16 @@ class OResult(C.Structure):
     ]
 
 
+class OResid(C.Structure):
+    _fields_ = [
+        ("num", C.c_double), ("den", C.c_double), ("resid", C.c_double), ("max_resid", C.c_double),
+        ("ijk_resid", C.c_int), ("ncells", C.c_int),
+    ]
+
+
+RESID_KINDS = {"C": 0, "S": 1, "PP": 2}
+
+
 def build_oracle(force: bool = False) -> Path:
     if force or not LIB.exists() or LIB.stat().st_mtime < (HERE / "leq_oracle.c").stat().st_mtime:
         subprocess.run(["make", "-s", "-C", str(HERE), "oracle"], check=True)
@@ -77,6 +87,8 @@ def lib() -> C.CDLL:
         l.oracle_leq_bicgs.argtypes = [C.POINTER(OGrid), C.c_char_p, C.c_char_p, C.c_double, C.c_int,
                                        _dp, _dp, _dp, _dp, C.POINTER(OResult), _dp]
         l.oracle_leq_bicgs.restype = C.c_int
+        l.oracle_calc_resid.argtypes = [C.POINTER(OGrid), C.c_int, _dp, _dp, _dp, C.POINTER(C.c_int), C.c_double,
+                                        _dp, C.c_double, C.c_double, C.c_int, C.POINTER(OResid)]
         _lib = l
     return _lib
 
@@ -152,13 +164,29 @@ def leq_bicgs(g, pc, cmethod, tol, itmax, var, a, b, noise, trace=False):
     return var, a, b, out
 
 
+def calc_resid(g, kind, var, a, b, flag, tol=0.0, ep_s=None, dil_ep_s=1.0e-4, norm=1.0, debug=True) -> dict:
+    """Oracle CALC_RESID_C / _S / _PP (kind "C", "S", "PP"); flag is MFIX's
+    FLAG array (int32, DIMENSION_3), ep_s the phase's EP_s or None (M = 0)."""
+    out = OResid()
+    fl = np.ascontiguousarray(flag, dtype=np.int32)
+    eps = None if ep_s is None else np.ascontiguousarray(ep_s, dtype=np.float64)
+    lib().oracle_calc_resid(C.byref(g), RESID_KINDS[kind], _p(np.ascontiguousarray(var, dtype=np.float64)),
+                            _p(np.ascontiguousarray(a, dtype=np.float64)),
+                            _p(np.ascontiguousarray(b, dtype=np.float64)),
+                            fl.ctypes.data_as(C.POINTER(C.c_int)), tol, _p(eps) if eps is not None else None,
+                            dil_ep_s, norm, int(debug), C.byref(out))
+    return {f: getattr(out, f) for f, _ in out._fields_}
+
+
 # ---- driver of the compiled reference (ref_harness.f90) --------------------
-OPS = {"bicgs": 1, "matvec": 2, "msolve": 3, "msolve1": 4, "dot": 5, "dot2": 6}
+OPS = {"bicgs": 1, "matvec": 2, "msolve": 3, "msolve1": 4, "dot": 5, "dot2": 6,
+       "resid_c": 7, "resid_s": 8, "resid_pp": 9}
 
 
 def run_reference(op, g: OGrid, a, b, x, pc="DIAG", cmethod="RSRS", tol=1e-10, itmax=500, seed=12345,
-                  workdir: Path | None = None):
-    """Run one reference routine through oracle/_ref/leq_ref_harness."""
+                  workdir: Path | None = None, resid: dict | None = None):
+    """Run one reference routine through oracle/_ref/leq_ref_harness.  The
+    residual ops take resid = {flag, debug, m, tol, norm, ep_s}."""
     if not REF_BIN.exists():
         raise FileNotFoundError(REF_BIN)
     n3 = n3_of(g)
@@ -175,6 +203,12 @@ def run_reference(op, g: OGrid, a, b, x, pc="DIAG", cmethod="RSRS", tol=1e-10, i
         f.write(np.ascontiguousarray(a, dtype=np.float64).tobytes())
         f.write(np.ascontiguousarray(b, dtype=np.float64).tobytes())
         f.write(np.ascontiguousarray(x, dtype=np.float64).tobytes())
+        if OPS[op] >= 7:
+            f.write(np.array([int(resid["debug"]), int(resid["m"])], dtype=np.int32).tobytes())
+            f.write(np.array([resid.get("tol", 0.0), resid.get("norm", 1.0)], dtype=np.float64).tobytes())
+            f.write(np.ascontiguousarray(resid["flag"], dtype=np.int32).tobytes())
+            eps = resid.get("ep_s")
+            f.write(np.ascontiguousarray(np.zeros(n3) if eps is None else eps, dtype=np.float64).tobytes())
     try:
         subprocess.run([str(REF_BIN), str(fin), str(fout)], check=True, capture_output=True,
                        env={**os.environ, "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS", "1")})

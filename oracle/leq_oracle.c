@@ -540,3 +540,150 @@ int oracle_leq_bicgs(const og_grid *g, const char *pc, const char *cmethod, doub
     free(alpha); free(omega); free(rho);
     return 0;
 }
+
+/* ---- residual norms: CALC_RESID_C / _S / _PP (model/calc_resid.f) --------- */
+
+typedef struct {
+    double num, den, resid, max_resid;
+    int ijk_resid, ncells;
+} og_resid;
+
+#define OG_UNDEFINED 9.87654321e31   /* UNDEFINED, model/param1_mod.f:18 */
+#define OG_LARGE_NUMBER 1.0e32       /* LARGE_NUMBER, model/param1_mod.f:23 */
+
+/* FLAG predicates, model/include/functions.inc:409-414 (FLUID_AT),
+ * :461-466 (FLOW_AT), :469-474 (WALL_AT). */
+static inline int og_fluid(int f) { return f == 1; }
+static inline int og_wall(int f) { return f >= 100; }
+static inline int og_flow(int f) { return f >= 10 && f <= 31; }
+
+/* Neighbours of cell (i,j,k) as SET_INDEX1A builds them
+ * (model/set_index1a.f:85-222): mp = IM, IP, JM, JP, KM, KP with the
+ * true-corner rule (a wall or flow cell on the first/last interior layer is
+ * its own neighbour across that face); nb = E, W, S, N, B, T, the mp
+ * neighbour unless that one is a wall cell, then the cell itself. */
+static void og_index1a(const og_dims *d, const int *flag, int i, int j, int k, int64_t mp[6], int64_t nb[6]) {
+    const og_grid *g = d->g;
+    const int64_t c = IJK(d, i, j, k);
+    const int wf = og_wall(flag[c]) || og_flow(flag[c]);
+    const int kmin1 = d->do_k ? 2 : 1, kmax1 = d->do_k ? d->nk - 1 : 1;
+    mp[0] = (wf && i == 2) ? c : IJK(d, nb_minus(i, d->ni, g->cyclic_x), j, k);
+    mp[1] = (wf && i == d->ni - 1) ? c : IJK(d, nb_plus(i, d->ni, g->cyclic_x), j, k);
+    mp[2] = (wf && j == 2) ? c : IJK(d, i, nb_minus(j, d->nj, g->cyclic_y), k);
+    mp[3] = (wf && j == d->nj - 1) ? c : IJK(d, i, nb_plus(j, d->nj, g->cyclic_y), k);
+    if (d->do_k) {
+        mp[4] = (wf && k == kmin1) ? c : IJK(d, i, j, nb_minus(k, d->nk, g->cyclic_z));
+        mp[5] = (wf && k == kmax1) ? c : IJK(d, i, j, nb_plus(k, d->nk, g->cyclic_z));
+    } else {
+        mp[4] = mp[5] = c;
+    }
+    /* E <- IP, W <- IM, S <- JM, N <- JP, B <- KM, T <- KP */
+    const int src[6] = {1, 0, 2, 3, 4, 5};
+    for (int q = 0; q < 6; ++q) nb[q] = og_wall(flag[mp[src[q]]]) ? c : mp[src[q]];
+}
+
+/* kind 0: CALC_RESID_C (model/calc_resid.f:16-178), kind 1: CALC_RESID_S
+ * (:195-371; tol = TOL, ep_s = EP_S(:,M) or NULL for M = 0, dil_ep_s =
+ * DIL_EP_s), kind 2: CALC_RESID_PP (:391-534, b only, norm = NORM).  Serial
+ * run: the loop over IJK is the storage order, IS_ON_myPE_wobnd
+ * (functions.inc:121-133) keeps the cells of the first interior layers
+ * ISTART1..IEND1 etc. (set_max2.f:53-69), and the DMP gathers reduce to the
+ * local values.  debug = DEBUG_RESID (model/run_mod.f:244).  With debug off,
+ * C and S return NUM and DEN only: RESID and IJK_RESID are left as 0 and
+ * MAX_RESID as -1 (the reference leaves RESID and IJK_RESID unset). */
+void oracle_calc_resid(const og_grid *g, int kind, const double *var, const double *a, const double *b,
+                       const int *flag, double tol, const double *ep_s, double dil_ep_s, double norm, int debug,
+                       og_resid *out) {
+    og_dims d = dims_of(g);
+    const int64_t n3 = d.n3;
+    double num = 0.0, den = 0.0, maxr = -1.0;
+    int ncells = 0;
+    int64_t ijkr = 1;                 /* 1-based IJK_RESID */
+    double *rijk = kind != 2 ? (double *)calloc((size_t)n3, sizeof(double)) : NULL;
+    const int kmin1 = d.do_k ? 2 : 1, kmax1 = d.do_k ? d.nk - 1 : 1;
+    for (int k = 1; k <= d.nk; ++k)
+        for (int i = 1; i <= d.ni; ++i)
+            for (int j = 1; j <= d.nj; ++j) {
+                const int64_t c = IJK(&d, i, j, k);
+                if (!(i >= 2 && i <= d.ni - 1 && j >= 2 && j <= d.nj - 1 && k >= kmin1 && k <= kmax1)) continue;
+                if (!og_fluid(flag[c])) continue;
+                if (kind == 2) {
+                    const double num1 = fabs(b[c]);
+                    if (num1 > maxr) { maxr = num1; ijkr = c + 1; }
+                    ncells += 1;
+                    num = num + num1;
+                    den = den + 1.0;
+                    continue;
+                }
+                if (kind == 1) {
+                    if (!(fabs(var[c]) > tol)) continue;
+                    if (ep_s && ep_s[c] <= dil_ep_s) continue;
+                }
+                int64_t mp[6], nb[6];
+                og_index1a(&d, flag, i, j, k, mp, nb);
+                int64_t e, w, s, n, bt, t;
+                if (kind == 0) { e = nb[0]; w = nb[1]; s = nb[2]; n = nb[3]; bt = nb[4]; t = nb[5]; }
+                else { w = mp[0]; e = mp[1]; s = mp[2]; n = mp[3]; bt = mp[4]; t = mp[5]; }
+                double num1 = b[c] - (A(0, c) * var[c] + A(1, c) * var[e] + A(-1, c) * var[w] +
+                                      A(2, c) * var[n] + A(-2, c) * var[s]);
+                if (d.do_k) num1 = num1 - (A(3, c) * var[t] + A(-3, c) * var[bt]);
+                num1 = fabs(num1);
+                const double den1 = fabs(A(0, c) * var[c]);
+                rijk[c] = num1;
+                ncells += 1;
+                num = num + num1;
+                den = den + den1;
+            }
+    out->num = num;
+    out->den = den;
+    out->ncells = ncells;
+    if (kind == 2) {
+        double r, m = maxr;
+        int64_t ij = ijkr;
+        if (den * norm > 0.0) {
+            r = num / (den * norm);
+            if (debug) m = ncells * maxr / (den * norm);
+        } else if (num == 0.0) {
+            r = 0.0;
+            if (debug) { m = 0.0; ij = 0; }
+        } else {
+            r = OG_LARGE_NUMBER;
+            if (debug) m = OG_LARGE_NUMBER;
+        }
+        out->resid = r;
+        out->max_resid = m;
+        out->ijk_resid = (int)ij;
+        return;
+    }
+    if (!debug) {
+        out->resid = 0.0;
+        out->max_resid = -1.0;
+        out->ijk_resid = 0;
+        free(rijk);
+        return;
+    }
+    /* first cell of the largest residual, from RESID_IJK(1) (:114-122) */
+    ijkr = 1;
+    maxr = rijk[0];
+    for (int k = 1; k <= d.nk; ++k)
+        for (int i = 1; i <= d.ni; ++i)
+            for (int j = 1; j <= d.nj; ++j) {
+                const int64_t c = IJK(&d, i, j, k);
+                if (!(i >= 2 && i <= d.ni - 1 && j >= 2 && j <= d.nj - 1 && k >= kmin1 && k <= kmax1)) continue;
+                if (rijk[c] > maxr) { ijkr = c + 1; maxr = rijk[c]; }
+            }
+    free(rijk);
+    if (den > 0.0) {
+        out->resid = num / den;
+        out->max_resid = ncells * maxr / den;
+        out->ijk_resid = (int)ijkr;
+    } else if (num == 0.0) {
+        out->resid = 0.0;
+        out->max_resid = 0.0;
+        out->ijk_resid = 0;
+    } else {
+        out->resid = OG_UNDEFINED;
+        out->max_resid = OG_UNDEFINED;
+        out->ijk_resid = (int)ijkr;
+    }
+}

@@ -12,12 +12,20 @@
 !   op 4  LEQ_MSOLVE1      model/leqsol_mod.f:681  (DIAG)
 !   op 5  dot_product_par  model/leqsol_mod.f:1094
 !   op 6  dot_product_par2 model/leqsol_mod.f:1210
+!   op 7  CALC_RESID_C     model/calc_resid.f:16
+!   op 8  CALC_RESID_S     model/calc_resid.f:195
+!   op 9  CALC_RESID_PP    model/calc_resid.f:391
 ! Input file (stream): 12 int32 (op, imax, jmax, kmax, no_k, cyc_x, cyc_y,
 ! cyc_z, is_serial, minimize_dotproducts, icheck_bicgs, itmax), int32 seed,
-! real64 tol, 4+4 chars pc/cmethod, then A(N3*7), B(N3), X(N3) real64.
+! real64 tol, 4+4 chars pc/cmethod, then A(N3*7), B(N3), X(N3) real64; ops
+! 7-9 then add int32 debug_resid, int32 phase M, real64 TOL (op 8), real64
+! NORM (op 9), FLAG(N3) int32 and EP_s(N3) real64 (phase M when M > 0).
+! For these ops the neighbour tables come from the reference's SET_INDEX1A
+! (model/set_index1a.f:34) on that FLAG, one class per cell.
 ! Output file (stream): int32 ier, int32 iter_tot, real64 dot results(2)
 ! (op 1: dres(2) = wall seconds of the LEQ_BICGS call),
-! X(N3), A(N3*7), B(N3), noise(N3).
+! X(N3), A(N3*7), B(N3), noise(N3).  Ops 7-9: ier = IJK_RESID, iter_tot =
+! 0, dres = (NUM, DEN), X(1:2) = (RESID, MAX_RESID).
 program ref_harness
   use param, only: dimension_3
   use compar
@@ -27,7 +35,14 @@ program ref_harness
   use parallel, only: is_serial, use_doloop
   use cutcell, only: re_indexing, cartesian_grid
   use funits, only: dmp_log
+  use run, only: debug_resid
+  use fldvar, only: rop_s, ro_s
+  use param, only: dimension_m
   implicit none
+  integer :: rhdr(2), mph, ijkr, ijkw, ijke, ijks, ijkn, ijkb, ijkt, imjk, ipjk, ijmk, ijpk, ijkm, ijkp
+  double precision :: rtol, rnorm, rnum, rden, rres, rmax
+  integer, allocatable :: flagin(:)
+  double precision, allocatable :: epsin(:), a3(:,:,:), b2(:,:)
   integer :: hdr(12), seed, op, itmax
   double precision :: tol
   character(len=4) :: pc, cmethod
@@ -155,7 +170,76 @@ program ref_harness
   read(10) amat
   read(10) bvec
   read(10) xvec
+  if (op >= 7) then
+     read(10) rhdr
+     read(10) rtol, rnorm
+     allocate(flagin(n3), epsin(n3))
+     read(10) flagin
+     read(10) epsin
+  end if
   close(10)
+
+  if (op >= 7) then
+     ! ---- cell flags, index maps and SET_INCREMENTS' tables (set_increments.f:
+     ! 52-112 IM1..KP1, :113-126 I_OF.., :128-178 classes via SET_INDEX1A)
+     debug_resid = rhdr(1) /= 0
+     mph = rhdr(2)
+     if (.not. allocated(ijksize3_all)) allocate(ijksize3_all(0:0))
+     ijksize3_all(0) = n3
+     ijkmax2 = n3
+     dimension_m = max(1, mph)
+     if (allocated(flag)) deallocate(flag)
+     allocate(flag(n3))
+     flag = flagin
+     allocate(i_of(n3), j_of(n3), k_of(n3))
+     allocate(im1(istart3:iend3), ip1(istart3:iend3), jm1(jstart3:jend3), jp1(jstart3:jend3))
+     allocate(km1(kstart3:kend3), kp1(kstart3:kend3))
+     do i = istart3, iend3
+        im1(i) = nbm(i, imin3, imax3, cyclic_x, imap_c); ip1(i) = nbp(i, imin3, imax3, cyclic_x, imap_c)
+     end do
+     do j = jstart3, jend3
+        jm1(j) = nbm(j, jmin3, jmax3, cyclic_y, jmap_c); jp1(j) = nbp(j, jmin3, jmax3, cyclic_y, jmap_c)
+     end do
+     do k = kstart3, kend3
+        if (do_k) then
+           km1(k) = nbm(k, kmin3, kmax3, cyclic_z, kmap_c); kp1(k) = nbp(k, kmin3, kmax3, cyclic_z, kmap_c)
+        else
+           km1(k) = k; kp1(k) = k
+        end if
+     end do
+     do k = kstart3, kend3
+        do i = istart3, iend3
+           do j = jstart3, jend3
+              ijk = ijk_array_of(i, j, k)
+              i_of(ijk) = i; j_of(ijk) = j; k_of(ijk) = k
+           end do
+        end do
+     end do
+     do k = kstart3, kend3
+        do i = istart3, iend3
+           do j = jstart3, jend3
+              ijk = ijk_array_of(i, j, k)
+              call set_index1a(i, j, k, ijk, imjk, ipjk, ijmk, ijpk, ijkm, ijkp, ijkw, ijke, ijks, ijkn, &
+                               ijkb, ijkt)
+              cell_class(ijk) = ijk
+              increment_for_mp(1, ijk) = imjk - ijk; increment_for_mp(2, ijk) = ipjk - ijk
+              increment_for_mp(3, ijk) = ijmk - ijk; increment_for_mp(4, ijk) = ijpk - ijk
+              increment_for_mp(5, ijk) = ijkm - ijk; increment_for_mp(6, ijk) = ijkp - ijk
+              increment_for_nb(1, ijk) = ijke - ijk; increment_for_nb(2, ijk) = ijkw - ijk
+              increment_for_nb(3, ijk) = ijks - ijk; increment_for_nb(4, ijk) = ijkn - ijk
+              increment_for_nb(5, ijk) = ijkb - ijk; increment_for_nb(6, ijk) = ijkt - ijk
+           end do
+        end do
+     end do
+     allocate(rop_s(n3, dimension_m), ro_s(n3, dimension_m))
+     ro_s = 1d0
+     rop_s = 0d0
+     if (mph > 0) rop_s(:, mph) = epsin
+     allocate(a3(n3, -3:3, 0:dimension_m), b2(n3, 0:dimension_m))
+     a3 = 0d0; b2 = 0d0
+     a3(:, :, mph) = amat
+     b2(:, mph) = bvec
+  end if
 
   iter_tot = 0
   ier = 0
@@ -189,6 +273,19 @@ program ref_harness
   case (6)
      dres = dot_product_par2(xvec, bvec, xvec, xvec)
      yvec = xvec
+  case (7, 8, 9)
+     rres = 0d0; rmax = 0d0; ijkr = 0
+     if (op == 7) then
+        call calc_resid_c(xvec, a3, b2, mph, rnum, rden, rres, rmax, ijkr)
+     else if (op == 8) then
+        call calc_resid_s(xvec, a3, b2, mph, rnum, rden, rres, rmax, ijkr, rtol)
+     else
+        call calc_resid_pp(b2, rnorm, rnum, rden, rres, rmax, ijkr)
+     end if
+     ier = ijkr
+     dres(1) = rnum; dres(2) = rden
+     yvec = 0d0
+     yvec(1) = rres; yvec(2) = rmax
   end select
 
   open(11, file=trim(fout), access='stream', form='unformatted', status='replace')
