@@ -208,6 +208,51 @@ def bench_line(args):
     return 0
 
 
+def bench_resid(args):
+    """CALC_RESID_C / _S / _PP (model/calc_resid.f) on the 256^3 void-fraction
+    pressure system (seed 2000): ghost cells walls (FLAG 100), interior fluid
+    (FLAG 1), VAR uniform in [-1,1].  One call = the residual pass, the
+    finish block and the copy of the result to the host (blocking)."""
+    from mfix_amd import leq, systems, _lib
+    n = args.n
+    a, b = systems.gen_system(systems.VOID_PRESSURE, n, n, n, 2000)
+    n3 = b.size
+    leq.grid_init(n, n, n)
+    f = np.full((n + 2, n + 2, n + 2), 100, dtype=np.int32)
+    f[1:-1, 1:-1, 1:-1] = 1
+    x = np.random.default_rng(5).uniform(-1, 1, n3)
+    da, db, dx = leq.DeviceArray.from_host(a), leq.DeviceArray.from_host(b), leq.DeviceArray.from_host(x)
+    dflag = leq.DeviceArray.from_host(f.reshape(-1))
+    per_cell = {"C": 7 * 8 + 8 + 8 + 4, "S": 7 * 8 + 8 + 8 + 4, "PP": 8 + 4}
+    res = {}
+    for kind in ("C", "S", "PP"):
+        def one():
+            return leq.calc_resid_dev(kind, b=db, flag=dflag, var=dx, a=da, tol=0.0, norm=1.0, debug=True)
+        for _ in range(max(1, args.warmup)):
+            one()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            out = one()
+        dt = (time.perf_counter() - t) / args.steps
+        res[kind] = dict(us=dt * 1e6, GBps=per_cell[kind] * n3 / dt / 1e9, resid=out["resid"],
+                         ncells=out["ncells"])
+    c = res["C"]
+    line = {
+        "metric": "CALC_RESID_C calls/s (residual norm of an assembled equation)",
+        "value": 1e6 / c["us"], "unit": "calls/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": c["us"] / 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"void-fraction pressure {n}^3 (N3={n3}), ghost walls, fluid interior",
+                   "per_kind": res},
+        "roofline": {"bound": "hbm", "kernel": "k_resid (CALC_RESID_C) + finish + result copy",
+                     "achieved": c["GBps"], "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": c["GBps"] * 1e9 / HBM_PEAK, "traffic": None},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line))
+    return 0
+
+
 def bench_serve(args):
     """Poisson request streams through the continuous-batching server: sides
     lognormal (median 64, sigma 0.5) clipped to [24,160], 60% upwind momentum
@@ -286,7 +331,7 @@ def main(argv=None):
     ap.add_argument("--cpu-iters", type=int, default=6, help="reference iterations timed for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
-    ap.add_argument("--workload", choices=["mixed", "batch", "serve", "line"], default="mixed",
+    ap.add_argument("--workload", choices=["mixed", "batch", "serve", "line", "resid"], default="mixed",
                     help="mixed: the 256^3 solve (headline); batch: 64 systems 16^3..128^3 in one batched "
                          "solve; serve: Poisson request streams through the continuous-batching server")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -318,6 +363,8 @@ def main(argv=None):
         return bench_serve(args)
     if args.workload == "line":
         return bench_line(args)
+    if args.workload == "resid":
+        return bench_resid(args)
     n = args.n
     t0 = time.time()
     a, b = systems.gen_system(systems.VOID_PRESSURE, n, n, n, 2000)

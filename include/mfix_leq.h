@@ -231,6 +231,46 @@ int mfix_calc_resid_pp(const double *b_m, const int *flag, double norm,
 int mfix_calc_resid_pp_dev(const double *d_b_m, const int *d_flag, double norm,
                            int debug_resid, mfix_leq_resid *out, void *stream);
 
+/* ---- the gas pressure-correction equation, assembled on the device -------- */
+/* Fields of INIT_AB_M + CONV_PP_G + SOURCE_PP_G for phase 0: DIMENSION_3
+ * device arrays in the storage contract above. */
+typedef struct mfix_pp_g_fields {
+    const double *rop_ge, *rop_gn, *rop_gt;   /* ROP_gE/N/T face bulk densities, model/mflux_mod.f:36-40 */
+    const double *ayz, *axz, *axy;            /* face areas, model/geometry_mod.f:206-210 */
+    const double *vol;                        /* cell volumes, model/geometry_mod.f:212 */
+    const double *rop_g, *rop_go;             /* ROP_g and its old-time value, model/fldvar_mod.f:38-41 */
+    const double *u_g, *v_g, *w_g;            /* gas velocities, model/fldvar_mod.f:87-111 */
+    const double *sum_r_g;                    /* SUM_R_g, model/rxns_mod.f:28 */
+    const double *d_e, *d_n, *d_t;            /* D_E(:,0), D_N(:,0), D_T(:,0), model/pgcor_mod.f:12-16 */
+    const int *flag;                          /* FLAG (1 fluid, 10/11 pressure flow, >= 100 walls) */
+    double odt;                               /* ODT = 1/DT, model/run_mod.f:54 */
+    int ijk_p_g;                              /* IJK_P_g (1-based reference pressure cell), 0 = UNDEFINED_I */
+} mfix_pp_g_fields;
+
+/* INIT_AB_M(A_M,B_M,IJKMAX2,0) + CONV_PP_G(A_M,B_M) + SOURCE_PP_G(A_M,B_M,
+ * B_MMAX) (model/init_ab_m.f:20, model/conv_pp_g.f:33, model/source_pp_g.f:18)
+ * for the gas phase with MMAX = 0, RO_G0 defined (incompressible: no
+ * COMPRESSIBLE_PP_G), no cut cells, no shear: writes A_M(:,:,0), B_M(:,0)
+ * and, on fluid cells, B_MMAX(:,0).  *n_singular (may be NULL) receives the
+ * number of fluid cells left with A0 = 0 and B != 0, where the reference
+ * writes an error message and carries on. */
+int mfix_assemble_pp_g_dev(const mfix_pp_g_fields *fields, double *d_a_m,
+                           double *d_b_m, double *d_b_mmax, int *n_singular,
+                           void *stream);
+
+/* SOLVE_PP_G(NORMG, RESG, IER) (model/solve_pp_g.f:25) for the gas phase:
+ * PP_G = 0, the assembly above, RESG = CALC_RESID_PP(B_M, NORMGloc) with
+ * NORMGloc = NORMG, or RESID of B_MMAX / 10 when NORMG = 0, then
+ * SOLVE_LIN_EQ('Pp_g', 1, PP_G, A_M, B_M, 0, ...) (METHOD 2 only) as
+ * LEQ_BICGS; mixed = 1 runs it in mixed precision.  d_pp_g receives PP_G,
+ * d_a_m/d_b_m the (row-scaled) system, ITER_TOT(1) is updated. */
+int mfix_solve_pp_g_dev(const mfix_pp_g_fields *fields, double normg,
+                        double *d_pp_g, double *d_a_m, double *d_b_m,
+                        double *d_b_mmax, int itmax, int method,
+                        const char *sweep, double tol, const char *pc,
+                        int mixed, double *resg, int *ier,
+                        mfix_leq_stats *stats, void *stream);
+
 /* ---- serving: continuous batching with HBM-budget admission ------------- */
 typedef struct mfix_leq_server mfix_leq_server;
 typedef struct mfix_leq_request {

@@ -53,6 +53,15 @@ class Resid(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+PP_FIELDS = ("rop_ge", "rop_gn", "rop_gt", "ayz", "axz", "axy", "vol", "rop_g", "rop_go", "u_g", "v_g", "w_g",
+             "sum_r_g", "d_e", "d_n", "d_t")
+
+
+class PPFields(C.Structure):
+    _fields_ = [(f, C.c_void_p) for f in PP_FIELDS] + [("flag", C.c_void_p), ("odt", C.c_double),
+                                                      ("ijk_p_g", C.c_int)]
+
+
 class BatchItem(C.Structure):
     _fields_ = [
         ("imax", C.c_int), ("jmax", C.c_int), ("kmax", C.c_int),
@@ -113,6 +122,10 @@ _SIGS = {
     "mfix_leq_set_graphs": (C.c_int, [C.c_int]),
     "mfix_leq_set_unit_diag": (C.c_int, [C.c_int]),
     "mfix_leq_set_shape": (C.c_int, [C.c_int] * 6),
+    "mfix_assemble_pp_g_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "mfix_solve_pp_g_dev": (C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                      C.c_int, C.c_char_p, C.c_double, C.c_char_p, C.c_int, C.c_void_p, C.c_void_p,
+                                      C.c_void_p, C.c_void_p]),
     "mfix_calc_resid_c": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
     "mfix_calc_resid_c_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
                                         C.c_void_p]),

@@ -177,5 +177,7 @@ int run_dot(const DGrid &g, const double *x, const double *y, double *d_out, hip
 int run_scale(const DGrid &g, double *a, double *b, int *zero_row, hipStream_t s);
 int run_copy(const double *src, double *dst, int64_t n, hipStream_t s);
 int autotune(int imax, int jmax, int kmax, int mixed, Shape *best);
+// CALC_RESID_PP of device arrays on the current grid (leq_resid.hip).
+int resid_pp(const double *b, const int *flag, double norm, int debug, mfix_leq_resid *out, hipStream_t s);
 
 }  // namespace mfix

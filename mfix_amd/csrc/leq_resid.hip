@@ -254,6 +254,10 @@ static ResidArgs resid_args(int kind, int debug, const double *var, const double
     return R;
 }
 
+int resid_pp(const double *b, const int *flag, double norm, int debug, mfix_leq_resid *out, hipStream_t s) {
+    return run_resid(resid_args(RESID_PP, debug, nullptr, nullptr, b, flag, 0.0, nullptr, 0.0, norm), out, s);
+}
+
 // Host arrays: staged through the context's device buffers.
 static int resid_host(int kind, const double *var, const double *a_m, const double *b_m, const int *flag, double tol,
                       const double *ep_s, double dil_ep_s, double norm, int debug, mfix_leq_resid *out) {

@@ -338,3 +338,43 @@ def calc_resid_dev(kind: str, *, b: "DeviceArray", flag: "DeviceArray", var: "De
     else:
         raise ValueError(f"unknown residual kind {kind!r}")
     return out.as_dict()
+
+
+# ---- the gas pressure-correction equation on the device ---------------------
+# (INIT_AB_M + CONV_PP_G + SOURCE_PP_G, SOLVE_PP_G: model/conv_pp_g.f,
+#  model/source_pp_g.f, model/solve_pp_g.f)
+
+def _pp_struct(fields: dict, flag: "DeviceArray", odt: float, ijk_p_g: int) -> "_lib.PPFields":
+    f = _lib.PPFields()
+    for k in _lib.PP_FIELDS:
+        setattr(f, k, fields[k].ptr)
+    f.flag = flag.ptr
+    f.odt = float(odt)
+    f.ijk_p_g = int(ijk_p_g)
+    return f
+
+
+def assemble_pp_g_dev(fields: dict, flag: "DeviceArray", odt: float, ijk_p_g: int, a: "DeviceArray",
+                      b: "DeviceArray", b_mmax: "DeviceArray") -> int:
+    """Assemble A_M(:,:,0), B_M(:,0) and B_MMAX(:,0) of the gas pressure
+    correction from resident fields (DeviceArrays keyed by the names of
+    mfix_pp_g_fields); returns the count of fluid cells with A0 = 0, B != 0."""
+    f = _pp_struct(fields, flag, odt, ijk_p_g)
+    nbad = C.c_int()
+    check(_lib_().mfix_assemble_pp_g_dev(C.byref(f), a.ptr, b.ptr, b_mmax.ptr, C.byref(nbad), None),
+          "mfix_assemble_pp_g_dev")
+    return nbad.value
+
+
+def solve_pp_g_dev(fields: dict, flag: "DeviceArray", odt: float, ijk_p_g: int, normg: float, pp_g: "DeviceArray",
+                   a: "DeviceArray", b: "DeviceArray", b_mmax: "DeviceArray", *, itmax: int = 500, method: int = 2,
+                   sweep: str = "RSRS", tol: float = 1e-10, pc: str = "DIAG", mixed: bool = False) -> dict:
+    """SOLVE_PP_G for the gas phase: returns RESG, IER and the solver stats."""
+    f = _pp_struct(fields, flag, odt, ijk_p_g)
+    resg = C.c_double()
+    ier = C.c_int()
+    st = Stats()
+    check(_lib_().mfix_solve_pp_g_dev(C.byref(f), normg, pp_g.ptr, a.ptr, b.ptr, b_mmax.ptr, itmax, method,
+                                      sweep.encode(), tol, pc.encode(), int(mixed), C.byref(resg), C.byref(ier),
+                                      C.byref(st), None), "mfix_solve_pp_g_dev")
+    return {"resg": resg.value, "ier": ier.value, "stats": st.as_dict()}

@@ -49,6 +49,15 @@ class OResid(C.Structure):
 
 RESID_KINDS = {"C": 0, "S": 1, "PP": 2}
 
+# field order of the pressure-correction assembly (og_pp_fields, ref_harness op 10)
+PP_FIELDS = ("rop_ge", "rop_gn", "rop_gt", "ayz", "axz", "axy", "vol", "rop_g", "rop_go", "u_g", "v_g", "w_g",
+             "sum_r_g", "d_e", "d_n", "d_t")
+
+
+class OPPFields(C.Structure):
+    _fields_ = [(f, _dp) for f in PP_FIELDS] + [("flag", C.POINTER(C.c_int)), ("odt", C.c_double),
+                                                ("ijk_p_g", C.c_int)]
+
 
 def build_oracle(force: bool = False) -> Path:
     if force or not LIB.exists() or LIB.stat().st_mtime < (HERE / "leq_oracle.c").stat().st_mtime:
@@ -89,6 +98,8 @@ def lib() -> C.CDLL:
         l.oracle_leq_bicgs.restype = C.c_int
         l.oracle_calc_resid.argtypes = [C.POINTER(OGrid), C.c_int, _dp, _dp, _dp, C.POINTER(C.c_int), C.c_double,
                                         _dp, C.c_double, C.c_double, C.c_int, C.POINTER(OResid)]
+        l.oracle_assemble_pp_g.argtypes = [C.POINTER(OGrid), C.POINTER(OPPFields), _dp, _dp, _dp]
+        l.oracle_assemble_pp_g.restype = C.c_int
         _lib = l
     return _lib
 
@@ -178,15 +189,30 @@ def calc_resid(g, kind, var, a, b, flag, tol=0.0, ep_s=None, dil_ep_s=1.0e-4, no
     return {f: getattr(out, f) for f, _ in out._fields_}
 
 
+def assemble_pp_g(g, fields: dict, flag, odt, ijk_p_g=0):
+    """Oracle INIT_AB_M + CONV_PP_G + SOURCE_PP_G for the gas phase; returns
+    (A_M, B_M, B_MMAX, number of fluid cells left with A0 = 0 and B != 0)."""
+    n3 = n3_of(g)
+    arrs = {k: np.ascontiguousarray(fields[k], dtype=np.float64) for k in PP_FIELDS}
+    fl = np.ascontiguousarray(flag, dtype=np.int32)
+    F = OPPFields(*[_p(arrs[k]) for k in PP_FIELDS], fl.ctypes.data_as(C.POINTER(C.c_int)), odt, int(ijk_p_g))
+    a = np.zeros(7 * n3)
+    b = np.zeros(n3)
+    bm = np.zeros(n3)
+    bad = lib().oracle_assemble_pp_g(C.byref(g), C.byref(F), _p(a), _p(b), _p(bm))
+    return a, b, bm, bad
+
+
 # ---- driver of the compiled reference (ref_harness.f90) --------------------
 OPS = {"bicgs": 1, "matvec": 2, "msolve": 3, "msolve1": 4, "dot": 5, "dot2": 6,
-       "resid_c": 7, "resid_s": 8, "resid_pp": 9}
+       "resid_c": 7, "resid_s": 8, "resid_pp": 9, "assemble_pp": 10}
 
 
 def run_reference(op, g: OGrid, a, b, x, pc="DIAG", cmethod="RSRS", tol=1e-10, itmax=500, seed=12345,
-                  workdir: Path | None = None, resid: dict | None = None):
+                  workdir: Path | None = None, resid: dict | None = None, pp: dict | None = None):
     """Run one reference routine through oracle/_ref/leq_ref_harness.  The
-    residual ops take resid = {flag, debug, m, tol, norm, ep_s}."""
+    residual ops take resid = {flag, debug, m, tol, norm, ep_s}, the
+    pressure-correction assembly op pp = {flag, odt, ijk_p_g, fields}."""
     if not REF_BIN.exists():
         raise FileNotFoundError(REF_BIN)
     n3 = n3_of(g)
@@ -203,7 +229,13 @@ def run_reference(op, g: OGrid, a, b, x, pc="DIAG", cmethod="RSRS", tol=1e-10, i
         f.write(np.ascontiguousarray(a, dtype=np.float64).tobytes())
         f.write(np.ascontiguousarray(b, dtype=np.float64).tobytes())
         f.write(np.ascontiguousarray(x, dtype=np.float64).tobytes())
-        if OPS[op] >= 7:
+        if OPS[op] == 10:
+            f.write(np.array([int(pp["ijk_p_g"])], dtype=np.int32).tobytes())
+            f.write(np.float64(pp["odt"]).tobytes())
+            f.write(np.ascontiguousarray(pp["flag"], dtype=np.int32).tobytes())
+            f.write(np.stack([np.ascontiguousarray(pp["fields"][k], dtype=np.float64) for k in PP_FIELDS],
+                             axis=1).T.copy().tobytes())
+        elif OPS[op] >= 7:
             f.write(np.array([int(resid["debug"]), int(resid["m"])], dtype=np.int32).tobytes())
             f.write(np.array([resid.get("tol", 0.0), resid.get("norm", 1.0)], dtype=np.float64).tobytes())
             f.write(np.ascontiguousarray(resid["flag"], dtype=np.int32).tobytes())

@@ -687,3 +687,127 @@ void oracle_calc_resid(const og_grid *g, int kind, const double *var, const doub
         out->ijk_resid = (int)ijkr;
     }
 }
+
+/* ---- pressure-correction assembly: INIT_AB_M + CONV_PP_G + SOURCE_PP_G ----
+ * Gas phase only (MMAX = 0), incompressible (RO_G0 defined: no
+ * COMPRESSIBLE_PP_G), no cut cells, no shear, no point or user sources.
+ *   INIT_AB_M    model/init_ab_m.f:20-78
+ *   CONV_PP_G    model/conv_pp_g.f:33-182 (gas loop :69-113), in its own
+ *                scatter form: a fluid cell writes its east/north/top face
+ *                coefficient into its own row and into the neighbour's
+ *                west/south/bottom entry
+ *   SOURCE_PP_G  model/source_pp_g.f:18-233 (gas terms :77-178, pressure
+ *                flow neighbours :193-212, reference cell :214-218)
+ * Fields are DIMENSION_3 arrays; ijk_p_g is 1-based (0 = UNDEFINED_I).
+ * Returns the number of fluid cells whose A0 vanished with B != 0 (the
+ * reference writes an error message there and carries on). */
+typedef struct {
+    const double *rop_ge, *rop_gn, *rop_gt, *ayz, *axz, *axy, *vol;
+    const double *rop_g, *rop_go, *u_g, *v_g, *w_g, *sum_r_g, *d_e, *d_n, *d_t;
+    const int *flag;
+    double odt;
+    int ijk_p_g;
+} og_pp_fields;
+
+static inline int og_pflow(int f) { return f == 10 || f == 11; }   /* P_FLOW_AT, functions.inc:417-423 */
+
+int oracle_assemble_pp_g(const og_grid *g, const og_pp_fields *F, double *a, double *b, double *bmmax) {
+    og_dims d = dims_of(g);
+    const int64_t n3 = d.n3;
+    const int *flag = F->flag;
+    int bad = 0;
+    /* INIT_AB_M */
+    for (int64_t c = 0; c < n3; ++c) {
+        for (int l = -3; l <= 3; ++l) A(l, c) = 0.0;
+        A(0, c) = -1.0;
+        b[c] = 0.0;
+    }
+    /* CONV_PP_G, gas loop */
+    for (int k = 1; k <= d.nk; ++k)
+        for (int i = 1; i <= d.ni; ++i)
+            for (int j = 1; j <= d.nj; ++j) {
+                const int64_t c = IJK(&d, i, j, k);
+                if (!og_fluid(flag[c])) continue;
+                int64_t mp[6], nb[6];
+                og_index1a(&d, flag, i, j, k, mp, nb);
+                double am = F->rop_ge[c] * F->ayz[c];
+                A(1, c) = am;
+                A(-1, mp[1]) = am;
+                am = F->rop_gn[c] * F->axz[c];
+                A(2, c) = am;
+                A(-2, mp[3]) = am;
+                if (d.do_k) {
+                    am = F->rop_gt[c] * F->axy[c];
+                    A(3, c) = am;
+                    A(-3, mp[5]) = am;
+                }
+                if (!og_fluid(flag[mp[0]])) A(-1, c) = F->rop_ge[mp[0]] * F->ayz[mp[0]];
+                if (!og_fluid(flag[mp[2]])) A(-2, c) = F->rop_gn[mp[2]] * F->axz[mp[2]];
+                if (d.do_k && !og_fluid(flag[mp[4]])) A(-3, c) = F->rop_gt[mp[4]] * F->axy[mp[4]];
+            }
+    /* SOURCE_PP_G, gas phase */
+    for (int k = 1; k <= d.nk; ++k)
+        for (int i = 1; i <= d.ni; ++i)
+            for (int j = 1; j <= d.nj; ++j) {
+                const int64_t c = IJK(&d, i, j, k);
+                if (og_fluid(flag[c])) {
+                    int64_t mp[6], nb[6];
+                    og_index1a(&d, flag, i, j, k, mp, nb);
+                    const int64_t im = mp[0], jm = mp[2], km = mp[4];
+                    const double bma = (F->rop_g[c] - F->rop_go[c]) * F->vol[c] * F->odt;
+                    const double bme = A(1, c) * F->u_g[c];
+                    const double bmw = A(-1, c) * F->u_g[im];
+                    const double bmn = A(2, c) * F->v_g[c];
+                    const double bms = A(-2, c) * F->v_g[jm];
+                    const double bmt = A(3, c) * F->w_g[c];
+                    const double bmb = A(-3, c) * F->w_g[km];
+                    const double bmr = F->sum_r_g[c] * F->vol[c];
+                    b[c] = -((-(bma + bme - bmw + bmn - bms + bmt - bmb)) + bmr);
+                    double m = fabs(bma);
+                    const double rest[7] = {fabs(bme), fabs(bmw), fabs(bmn), fabs(bms), fabs(bmt), fabs(bmb),
+                                            fabs(bmr)};
+                    for (int q = 0; q < 7; ++q) m = rest[q] > m ? rest[q] : m;
+                    bmmax[c] = m;
+                    A(1, c) = A(1, c) * F->d_e[c];
+                    A(-1, c) = A(-1, c) * F->d_e[im];
+                    A(2, c) = A(2, c) * F->d_n[c];
+                    A(-2, c) = A(-2, c) * F->d_n[jm];
+                    A(3, c) = A(3, c) * F->d_t[c];
+                    A(-3, c) = A(-3, c) * F->d_t[km];
+                    A(0, c) = -(A(1, c) + A(-1, c) + A(2, c) + A(-2, c) + A(3, c) + A(-3, c));
+                    if (fabs(A(0, c)) < 1.0e-15) {   /* SMALL_NUMBER, param1_mod.f:24 */
+                        if (fabs(b[c]) < 1.0e-15) {
+                            A(0, c) = -1.0;
+                            b[c] = 0.0;
+                        } else {
+                            bad += 1;
+                        }
+                    }
+                } else {
+                    for (int l = -3; l <= 3; ++l) A(l, c) = 0.0;
+                    A(0, c) = -1.0;
+                    b[c] = 0.0;
+                }
+            }
+    for (int k = 1; k <= d.nk; ++k)
+        for (int i = 1; i <= d.ni; ++i)
+            for (int j = 1; j <= d.nj; ++j) {
+                const int64_t c = IJK(&d, i, j, k);
+                if (!og_fluid(flag[c])) continue;
+                int64_t mp[6], nb[6];
+                og_index1a(&d, flag, i, j, k, mp, nb);
+                if (og_pflow(flag[mp[0]])) A(-1, c) = 0.0;
+                if (og_pflow(flag[mp[1]])) A(1, c) = 0.0;
+                if (og_pflow(flag[mp[2]])) A(-2, c) = 0.0;
+                if (og_pflow(flag[mp[3]])) A(2, c) = 0.0;
+                if (og_pflow(flag[mp[4]])) A(-3, c) = 0.0;
+                if (og_pflow(flag[mp[5]])) A(3, c) = 0.0;
+            }
+    if (F->ijk_p_g > 0) {
+        const int64_t c = F->ijk_p_g - 1;
+        b[c] = 0.0;
+        for (int l = -3; l <= 3; ++l) A(l, c) = 0.0;
+        A(0, c) = -1.0;
+    }
+    return bad;
+}

@@ -15,6 +15,9 @@
 !   op 7  CALC_RESID_C     model/calc_resid.f:16
 !   op 8  CALC_RESID_S     model/calc_resid.f:195
 !   op 9  CALC_RESID_PP    model/calc_resid.f:391
+!   op 10 INIT_AB_M + CONV_PP_G + SOURCE_PP_G (gas phase, MMAX = 0)
+!                          model/init_ab_m.f:20, model/conv_pp_g.f:33,
+!                          model/source_pp_g.f:18
 ! Input file (stream): 12 int32 (op, imax, jmax, kmax, no_k, cyc_x, cyc_y,
 ! cyc_z, is_serial, minimize_dotproducts, icheck_bicgs, itmax), int32 seed,
 ! real64 tol, 4+4 chars pc/cmethod, then A(N3*7), B(N3), X(N3) real64; ops
@@ -26,6 +29,10 @@
 ! (op 1: dres(2) = wall seconds of the LEQ_BICGS call),
 ! X(N3), A(N3*7), B(N3), noise(N3).  Ops 7-9: ier = IJK_RESID, iter_tot =
 ! 0, dres = (NUM, DEN), X(1:2) = (RESID, MAX_RESID).
+! Op 10 adds, after X: int32 IJK_P_g (0 = undefined), real64 ODT, FLAG(N3)
+! int32 and 16 real64 arrays of N3: ROP_gE, ROP_gN, ROP_gT, AYZ, AXZ, AXY,
+! VOL, ROP_g, ROP_go, U_g, V_g, W_g, SUM_R_g, D_E(:,0), D_N(:,0), D_T(:,0);
+! it returns the assembled A_M/B_M in A/B and B_MMAX in X.
 program ref_harness
   use param, only: dimension_3
   use compar
@@ -38,7 +45,18 @@ program ref_harness
   use run, only: debug_resid
   use fldvar, only: rop_s, ro_s
   use param, only: dimension_m
+  use mflux, only: rop_ge, rop_gn, rop_gt
+  use fldvar, only: rop_g, rop_go, u_g, v_g, w_g
+  use rxns, only: sum_r_g
+  use pgcor, only: d_e, d_n, d_t
+  use run, only: odt, shear
+  use physprop, only: mmax, ro_g0
+  use bc, only: ijk_p_g
+  use param1, only: undefined_i
   implicit none
+  integer :: phdr(1)
+  double precision :: podt
+  double precision, allocatable :: pf(:,:), bmm(:,:)
   integer :: rhdr(2), mph, ijkr, ijkw, ijke, ijks, ijkn, ijkb, ijkt, imjk, ipjk, ijmk, ijpk, ijkm, ijkp
   double precision :: rtol, rnorm, rnum, rden, rres, rmax
   integer, allocatable :: flagin(:)
@@ -170,7 +188,14 @@ program ref_harness
   read(10) amat
   read(10) bvec
   read(10) xvec
-  if (op >= 7) then
+  if (op == 10) then
+     read(10) phdr
+     read(10) podt
+     allocate(flagin(n3), pf(n3, 16))
+     read(10) flagin
+     read(10) pf
+  end if
+  if (op >= 7 .and. op <= 9) then
      read(10) rhdr
      read(10) rtol, rnorm
      allocate(flagin(n3), epsin(n3))
@@ -180,6 +205,12 @@ program ref_harness
   close(10)
 
   if (op >= 7) then
+     if (op == 10) then
+        rhdr = 0
+        rtol = 0d0; rnorm = 1d0
+        allocate(epsin(n3))
+        epsin = 0d0
+     end if
      ! ---- cell flags, index maps and SET_INCREMENTS' tables (set_increments.f:
      ! 52-112 IM1..KP1, :113-126 I_OF.., :128-178 classes via SET_INDEX1A)
      debug_resid = rhdr(1) /= 0
@@ -273,6 +304,33 @@ program ref_harness
   case (6)
      dres = dot_product_par2(xvec, bvec, xvec, xvec)
      yvec = xvec
+  case (10)
+     allocate(rop_ge(n3), rop_gn(n3), rop_gt(n3), ayz(n3), axz(n3), axy(n3), vol(n3))
+     allocate(rop_g(n3), rop_go(n3), u_g(n3), v_g(n3), w_g(n3), sum_r_g(n3))
+     allocate(d_e(n3, 0:dimension_m), d_n(n3, 0:dimension_m), d_t(n3, 0:dimension_m))
+     allocate(bmm(n3, 0:dimension_m))
+     rop_ge = pf(:, 1); rop_gn = pf(:, 2); rop_gt = pf(:, 3)
+     ayz = pf(:, 4); axz = pf(:, 5); axy = pf(:, 6); vol = pf(:, 7)
+     rop_g = pf(:, 8); rop_go = pf(:, 9); u_g = pf(:, 10); v_g = pf(:, 11); w_g = pf(:, 12)
+     sum_r_g = pf(:, 13)
+     d_e = 0d0; d_n = 0d0; d_t = 0d0
+     d_e(:, 0) = pf(:, 14); d_n(:, 0) = pf(:, 15); d_t(:, 0) = pf(:, 16)
+     odt = podt
+     shear = .false.
+     mmax = 0
+     ro_g0 = 1d0
+     if (phdr(1) > 0) then
+        ijk_p_g = phdr(1)
+     else
+        ijk_p_g = undefined_i
+     end if
+     bmm = 0d0
+     call init_ab_m(a3, b2, n3, 0)
+     call conv_pp_g(a3, b2)
+     call source_pp_g(a3, b2, bmm)
+     amat = a3(:, :, 0)
+     bvec = b2(:, 0)
+     yvec = bmm(:, 0)
   case (7, 8, 9)
      rres = 0d0; rmax = 0d0; ijkr = 0
      if (op == 7) then
