@@ -253,6 +253,75 @@ def bench_resid(args):
     return 0
 
 
+def bench_pp(args):
+    """The gas pressure-correction equation of one SIMPLE iteration assembled
+    and solved on the device (mfix_solve_pp_g_dev: INIT_AB_M + CONV_PP_G +
+    SOURCE_PP_G, the residual normalisation and LEQ_BICGS with MFIX's
+    defaults for equation 1: LINE, RSRS, TOL 1e-4, 20 iterations, FP64) on
+    seeded fields of an n^3 box with a pressure-outflow face.  Beside it,
+    the host-array path a CPU-assembled A_M/B_M takes (mfix_leq_bicgs: A_M,
+    B_M and PP_G over PCIe, the same solve, PP_G back; the CPU assembly
+    itself is not timed)."""
+    from mfix_amd import leq, systems, _lib
+    n = args.n
+    fh, flag_h = systems.pp_g_fields(n, n, n, 2024)
+    n3 = flag_h.size
+    leq.grid_init(n, n, n)
+    fields = {k: leq.DeviceArray.from_host(v) for k, v in fh.items()}
+    flag = leq.DeviceArray.from_host(flag_h)
+    da, db, dbm, dpp = leq.DeviceArray(7 * n3), leq.DeviceArray(n3), leq.DeviceArray(n3), leq.DeviceArray(n3)
+    L = _lib.load()
+    for _ in range(3):
+        leq.assemble_pp_g_dev(fields, flag, 100.0, 0, da, db, dbm)
+    _lib.check(L.mfix_leq_sync(), "sync")
+    reps = 20
+    t = time.perf_counter()
+    for _ in range(reps):
+        leq.assemble_pp_g_dev(fields, flag, 100.0, 0, da, db, dbm)
+    asm_s = (time.perf_counter() - t) / reps
+    asm_bytes = (16 * 8 + 4 + 7 * 8 + 8 + 8) * n3
+    kw = dict(itmax=20, sweep="RSRS", tol=1e-4, pc="LINE", mixed=False)
+    for _ in range(args.warmup):
+        leq.solve_pp_g_dev(fields, flag, 100.0, 0, 0.0, dpp, da, db, dbm, **kw)
+    times = []
+    for _ in range(args.steps):
+        _lib.check(L.mfix_leq_sync(), "sync")
+        t = time.perf_counter()
+        out = leq.solve_pp_g_dev(fields, flag, 100.0, 0, 0.0, dpp, da, db, dbm, **kw)
+        times.append(time.perf_counter() - t)
+    dev_ms = 1e3 * sum(times) / len(times)
+    # host path: the assembled system as a CPU assembly would hand it over
+    leq.assemble_pp_g_dev(fields, flag, 100.0, 0, da, db, dbm)
+    a_h, b_h = da.download(), db.download()
+    _lib.check(L.mfix_leq_set_write_back(0), "write_back")
+    host_times = []
+    for _ in range(args.steps):
+        a_c, b_c, x_c = a_h.copy(), b_h.copy(), np.zeros(n3)
+        t = time.perf_counter()
+        ier_h = leq.leq_bicgs("Pp_g", 1, x_c, a_c, b_c, "RSRS", 1e-4, "LINE", 20)
+        host_times.append(time.perf_counter() - t)
+    _lib.check(L.mfix_leq_set_write_back(1), "write_back")
+    host_ms = 1e3 * sum(host_times) / len(host_times)
+    line = {
+        "metric": "SOLVE_PP_G/s (gas pressure correction assembled and solved on the device)",
+        "value": 1e3 / dev_ms, "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dev_ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"gas pressure correction {n}^3 (N3={n3}), pressure-outflow face, "
+                               "LINE RSRS, TOL 1e-4, ITMAX 20 (MFIX defaults, equation 1)",
+                   "iterations": out["stats"]["iterations"], "ier": out["ier"], "resg": out["resg"],
+                   "assembly_us": asm_s * 1e6, "assembly_alg_GBps": asm_bytes / asm_s / 1e9,
+                   "host_path_ms": host_ms, "host_path_ier": ier_h,
+                   "host_copy_bytes_avoided": 9 * 8 * n3},
+        "roofline": {"bound": "hbm", "kernel": "k_assemble_pp_g (+ counter read-back)",
+                     "achieved": asm_bytes / asm_s / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": asm_bytes / asm_s / HBM_PEAK, "traffic": None},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line))
+    return 0
+
+
 def bench_serve(args):
     """Poisson request streams through the continuous-batching server: sides
     lognormal (median 64, sigma 0.5) clipped to [24,160], 60% upwind momentum
@@ -331,7 +400,7 @@ def main(argv=None):
     ap.add_argument("--cpu-iters", type=int, default=6, help="reference iterations timed for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
-    ap.add_argument("--workload", choices=["mixed", "batch", "serve", "line", "resid"], default="mixed",
+    ap.add_argument("--workload", choices=["mixed", "batch", "serve", "line", "resid", "pp"], default="mixed",
                     help="mixed: the 256^3 solve (headline); batch: 64 systems 16^3..128^3 in one batched "
                          "solve; serve: Poisson request streams through the continuous-batching server")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -365,6 +434,8 @@ def main(argv=None):
         return bench_line(args)
     if args.workload == "resid":
         return bench_resid(args)
+    if args.workload == "pp":
+        return bench_pp(args)
     n = args.n
     t0 = time.time()
     a, b = systems.gen_system(systems.VOID_PRESSURE, n, n, n, 2000)

@@ -16,6 +16,7 @@ enum ResidKind : int { RESID_C = 0, RESID_S = 1, RESID_PP = 2 };
 constexpr double kUndefined = 9.87654321e31;   // UNDEFINED, model/param1_mod.f:18
 constexpr double kLargeNumber = 1.0e32;        // LARGE_NUMBER, model/param1_mod.f:23
 constexpr int kResidBlock = 256;
+constexpr int kResidCells = 8;                 // cells per thread: 2048 per block and partial
 constexpr int kResidFinish = 1024;
 constexpr int kResidVals = 5;                  // num, den, ncells, best value, best index
 
@@ -48,7 +49,6 @@ __global__ void __launch_bounds__(kResidBlock) k_resid(ResidArgs R) {
     __shared__ double lds[(kResidBlock / 64) * 3];
     __shared__ double lv[kResidBlock / 64], li[kResidBlock / 64];
     const DGrid &g = R.g;
-    const int64_t c = (int64_t)blockIdx.x * kResidBlock + threadIdx.x;
     // candidates must beat the scan's start value: RESID_IJK(1) = 0 for C
     // and S (:114-115), -1 for PP (:423)
     const double init = R.kind == RESID_PP ? -1.0 : 0.0;
@@ -59,52 +59,45 @@ __global__ void __launch_bounds__(kResidBlock) k_resid(ResidArgs R) {
     const auto *__restrict__ var = to_global(R.var);
     const auto *__restrict__ b = to_global(R.b);
     const auto *__restrict__ flag = to_global(R.flag);
-    if (c < g.n3 && is_interior(g, c) && fluid_at(flag[c])) {   // IS_ON_myPE_wobnd, FLUID_AT
+    const int64_t c0 = (int64_t)blockIdx.x * (kResidBlock * kResidCells) + threadIdx.x;
+    for (int q = 0; q < kResidCells; ++q) {
+        const int64_t c = c0 + (int64_t)q * kResidBlock;   // increasing per thread
+        if (c >= g.n3 || !is_interior(g, c) || !fluid_at(flag[c])) continue;   // IS_ON_myPE_wobnd, FLUID_AT
+        double num1, den1;
         if (R.kind == RESID_PP) {
-            const double num1 = fabs(b[c]);
-            sums[0] = num1;
-            sums[1] = 1.0;
-            sums[2] = 1.0;
-            if (num1 > init) {
-                bv = num1;
-                bi = (double)c;
-            }
+            num1 = fabs(b[c]);
+            den1 = 1.0;
         } else {
             const double vc = var[c];
-            bool take = true;
             if (R.kind == RESID_S) {
-                take = fabs(vc) > R.tol;
-                if (take && R.ep_s) take = !(to_global(R.ep_s)[c] <= R.dil_ep_s);
+                if (!(fabs(vc) > R.tol)) continue;
+                if (R.ep_s && to_global(R.ep_s)[c] <= R.dil_ep_s) continue;
             }
-            if (take) {
-                const Nb n = neighbours(g, c);
-                int64_t e = n.ip, w = n.im, no = n.jp, so = n.jm, t = n.kp, bo = n.km;
-                if (R.kind == RESID_C) {
-                    // EAST_OF etc.: a wall neighbour is replaced by the cell
-                    // itself (model/set_index1a.f:93-97 and the like)
-                    e = wall_at(flag[e]) ? c : e;
-                    w = wall_at(flag[w]) ? c : w;
-                    no = wall_at(flag[no]) ? c : no;
-                    so = wall_at(flag[so]) ? c : so;
-                    t = wall_at(flag[t]) ? c : t;
-                    bo = wall_at(flag[bo]) ? c : bo;
-                }
-                const int64_t n3 = g.n3;
-                // B - (A0 V + AE VE + AW VW + AN VN + AS VS), then the top and
-                // bottom pair (calc_resid.f:94-103), in that order
-                double num1 = b[c] - (a[3 * n3 + c] * vc + a[4 * n3 + c] * var[e] + a[2 * n3 + c] * var[w] +
-                                      a[5 * n3 + c] * var[no] + a[1 * n3 + c] * var[so]);
-                if (g.do_k) num1 = num1 - (a[6 * n3 + c] * var[t] + a[0 * n3 + c] * var[bo]);
-                num1 = fabs(num1);
-                sums[0] = num1;
-                sums[1] = fabs(a[3 * n3 + c] * vc);
-                sums[2] = 1.0;
-                if (num1 > init) {
-                    bv = num1;
-                    bi = (double)c;
-                }
+            const Nb n = neighbours(g, c);
+            int64_t e = n.ip, w = n.im, no = n.jp, so = n.jm, t = n.kp, bo = n.km;
+            if (R.kind == RESID_C) {
+                // EAST_OF etc.: a wall neighbour is replaced by the cell
+                // itself (model/set_index1a.f:93-97 and the like)
+                e = wall_at(flag[e]) ? c : e;
+                w = wall_at(flag[w]) ? c : w;
+                no = wall_at(flag[no]) ? c : no;
+                so = wall_at(flag[so]) ? c : so;
+                t = wall_at(flag[t]) ? c : t;
+                bo = wall_at(flag[bo]) ? c : bo;
             }
+            const int64_t n3 = g.n3;
+            // B - (A0 V + AE VE + AW VW + AN VN + AS VS), then the top and
+            // bottom pair (calc_resid.f:94-103), in that order
+            num1 = b[c] - (a[3 * n3 + c] * vc + a[4 * n3 + c] * var[e] + a[2 * n3 + c] * var[w] +
+                           a[5 * n3 + c] * var[no] + a[1 * n3 + c] * var[so]);
+            if (g.do_k) num1 = num1 - (a[6 * n3 + c] * var[t] + a[0 * n3 + c] * var[bo]);
+            num1 = fabs(num1);
+            den1 = fabs(a[3 * n3 + c] * vc);
         }
+        sums[0] += num1;
+        sums[1] += den1;
+        sums[2] += 1.0;
+        if (num1 > init) better(bv, bi, num1, (double)c);
     }
     block_sum<kResidBlock, 3>(sums, lds);
     // block candidate: wave shuffles, then the waves in order
@@ -215,7 +208,7 @@ __global__ void __launch_bounds__(kResidFinish) k_resid_finish(ResidArgs R) {
 static int run_resid(ResidArgs R, mfix_leq_resid *out, hipStream_t s) {
     Context &C = ctx();
     int rc;
-    R.nblk = (R.g.n3 + kResidBlock - 1) / kResidBlock;
+    R.nblk = (R.g.n3 + kResidBlock * kResidCells - 1) / (kResidBlock * kResidCells);
     if ((rc = C.partials.ensure(sizeof(double) * kResidVals * (size_t)R.nblk + 256))) return rc;
     R.partials = (double *)C.partials.ptr;
     R.out = (mfix_leq_resid *)((char *)C.partials.ptr + sizeof(double) * kResidVals * (size_t)R.nblk);

@@ -79,3 +79,29 @@ def serving_requests(rate: float, n: int, seed: int = 4000, median: float = 64.0
     tols = np.where(rng.uniform(size=n) < 0.5, 1e-8, 1e-10)
     return [dict(t=float(t[q]), side=int(sides[q]), kind=int(kinds[q]), tol=float(tols[q]), seed=seed + q)
             for q in range(n)]
+
+
+def pp_g_fields(imax: int, jmax: int, kmax: int, seed: int) -> tuple[dict, np.ndarray]:
+    """Seeded fields of a gas pressure-correction assembly on a closed box
+    with a pressure-outflow face: face densities ROP_gE/N/T, uniform 1 cm
+    cells, ROP_g near 1 with a small change since the last step, velocities
+    in [-1,1], a small mass source, D_E/D_N/D_T in [0.1,1]; FLAG 100 on the
+    ghost layer, 20 on the I- face (inflow), 11 on the I+ face (pressure
+    outflow), 1 inside."""
+    ni, nj, nk = imax + 2, jmax + 2, kmax + 2
+    n3 = ni * nj * nk
+    rng = np.random.default_rng(seed)
+    dx = 0.01
+    f = {
+        "rop_ge": rng.uniform(0.5, 1.5, n3), "rop_gn": rng.uniform(0.5, 1.5, n3), "rop_gt": rng.uniform(0.5, 1.5, n3),
+        "ayz": np.full(n3, dx * dx), "axz": np.full(n3, dx * dx), "axy": np.full(n3, dx * dx),
+        "vol": np.full(n3, dx ** 3), "rop_g": rng.uniform(0.9, 1.1, n3), "u_g": rng.uniform(-1, 1, n3),
+        "v_g": rng.uniform(-1, 1, n3), "w_g": rng.uniform(-1, 1, n3), "sum_r_g": rng.uniform(-1e-3, 1e-3, n3),
+        "d_e": rng.uniform(0.1, 1.0, n3), "d_n": rng.uniform(0.1, 1.0, n3), "d_t": rng.uniform(0.1, 1.0, n3),
+    }
+    f["rop_go"] = f["rop_g"] * rng.uniform(0.99, 1.01, n3)
+    flag = np.full((nk, ni, nj), 100, dtype=np.int32)
+    flag[1:-1, 1:-1, 1:-1] = 1
+    flag[1:-1, 0, 1:-1] = 20
+    flag[1:-1, ni - 1, 1:-1] = 11
+    return f, flag.reshape(-1)
