@@ -322,6 +322,98 @@ def bench_pp(args):
     return 0
 
 
+def bench_phi(args):
+    """A gas-phase scalar transport equation of one SIMPLE iteration on the
+    device, in SOLVE_SCALAR_EQ's order (model/solve_scalar_eq.f): INIT_AB_M +
+    CONV_DIF_PHI (first-order upwinding), SOURCE_PHI, CALC_RESID_S,
+    UNDER_RELAX_S (UR_FAC(9) = 0.8) and LEQ_BICGS with MFIX's defaults for
+    equation 9 (LINE, RSRS, TOL 1e-4, 15 iterations, FP64), on seeded fields
+    of an n^3 box (systems.phi_fields).  BC_PHI is not applied (it stays with
+    the caller).  Each kernel is also timed alone; beside the chain, the
+    host-array path a CPU-assembled A_M/B_M takes (A_M, B_M and PHI over PCIe,
+    the same solve, PHI back)."""
+    from mfix_amd import leq, systems, _lib
+    n = args.n
+    fh, flag_h = systems.phi_fields(n, n, n, 2025)
+    n3 = flag_h.size
+    leq.grid_init(n, n, n)
+    fields = {k: leq.DeviceArray.from_host(v) for k, v in fh.items()}
+    flag = leq.DeviceArray.from_host(flag_h)
+    da, db = leq.DeviceArray(7 * n3), leq.DeviceArray(n3)
+    L = _lib.load()
+    src = (fields["s_p"], fields["s_c"], fields["ep"], fields["phi"])
+    stages = {
+        "conv_dif_phi": (lambda: leq.conv_dif_phi_dev(fields, flag, da, db, sync=False), 148),
+        "source_phi": (lambda: leq.source_phi_dev(*src, flag, da, db, sync=False), 108),
+        "under_relax_s": (lambda: leq.under_relax_s_dev(fields["phi"], flag, 1.0, da, db, sync=False), 44),
+    }
+    per = {}
+    reps = 20
+    for name, (fn, bpc) in stages.items():
+        for _ in range(3):
+            fn()
+        _lib.check(L.mfix_leq_sync(), "sync")
+        t = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        _lib.check(L.mfix_leq_sync(), "sync")
+        sec = (time.perf_counter() - t) / reps
+        per[name] = {"us": sec * 1e6, "bytes_per_cell": bpc, "alg_GBps": bpc * n3 / sec / 1e9,
+                     "frac": bpc * n3 / sec / HBM_PEAK}
+    kw = dict(pc="LINE", cmethod="RSRS", tol=1e-4, itmax=15)
+    phi0 = fh["phi"]
+
+    def chain():
+        leq.conv_dif_phi_dev(fields, flag, da, db, sync=False)
+        leq.source_phi_dev(*src, flag, da, db, sync=False)
+        r = leq.calc_resid_dev("S", var=fields["phi"], a=da, b=db, flag=flag, debug=False)
+        leq.under_relax_s_dev(fields["phi"], flag, 0.8, da, db, sync=False)
+        return r, leq.bicgs_dev(fields["phi"], da, db, **kw)
+    for _ in range(args.warmup):
+        fields["phi"].upload(phi0)
+        chain()
+    times = []
+    for _ in range(args.steps):
+        fields["phi"].upload(phi0)
+        _lib.check(L.mfix_leq_sync(), "sync")
+        t = time.perf_counter()
+        r, out = chain()
+        times.append(time.perf_counter() - t)
+    dev_ms = 1e3 * sum(times) / len(times)
+    # host path: the assembled, under-relaxed system as a CPU assembly would hand it over
+    fields["phi"].upload(phi0)
+    leq.conv_dif_phi_dev(fields, flag, da, db)
+    leq.source_phi_dev(*src, flag, da, db)
+    leq.under_relax_s_dev(fields["phi"], flag, 0.8, da, db)
+    a_h, b_h = da.download(), db.download()
+    _lib.check(L.mfix_leq_set_write_back(0), "write_back")
+    host_times = []
+    for _ in range(args.steps):
+        a_c, b_c, x_c = a_h.copy(), b_h.copy(), phi0.copy()
+        t = time.perf_counter()
+        ier_h = leq.leq_bicgs("Scalar 1", 9, x_c, a_c, b_c, "RSRS", 1e-4, "LINE", 15)
+        host_times.append(time.perf_counter() - t)
+    _lib.check(L.mfix_leq_set_write_back(1), "write_back")
+    host_ms = 1e3 * sum(host_times) / len(host_times)
+    conv = per["conv_dif_phi"]
+    line = {
+        "metric": "scalar transport equations/s (assembled, under-relaxed and solved on the device)",
+        "value": 1e3 / dev_ms, "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dev_ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"gas-phase scalar {n}^3 (N3={n3}), first-order upwinding, UR 0.8, "
+                               "LINE RSRS, TOL 1e-4, ITMAX 15 (MFIX defaults, equation 9)",
+                   "iterations": out.stats["iterations"], "ier": out.ier, "resid_num": r["num"],
+                   "resid_den": r["den"], "kernels": per, "host_path_ms": host_ms, "host_path_ier": ier_h,
+                   "host_copy_bytes_avoided": 10 * 8 * n3},
+        "roofline": {"bound": "hbm", "kernel": "k_conv_dif_phi", "achieved": conv["alg_GBps"],
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": conv["frac"], "traffic": None},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line))
+    return 0
+
+
 def bench_serve(args):
     """Poisson request streams through the continuous-batching server: sides
     lognormal (median 64, sigma 0.5) clipped to [24,160], 60% upwind momentum
@@ -400,7 +492,7 @@ def main(argv=None):
     ap.add_argument("--cpu-iters", type=int, default=6, help="reference iterations timed for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
-    ap.add_argument("--workload", choices=["mixed", "batch", "serve", "line", "resid", "pp"], default="mixed",
+    ap.add_argument("--workload", choices=["mixed", "batch", "serve", "line", "resid", "pp", "phi"], default="mixed",
                     help="mixed: the 256^3 solve (headline); batch: 64 systems 16^3..128^3 in one batched "
                          "solve; serve: Poisson request streams through the continuous-batching server")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -436,6 +528,8 @@ def main(argv=None):
         return bench_resid(args)
     if args.workload == "pp":
         return bench_pp(args)
+    if args.workload == "phi":
+        return bench_phi(args)
     n = args.n
     t0 = time.time()
     a, b = systems.gen_system(systems.VOID_PRESSURE, n, n, n, 2000)

@@ -271,6 +271,45 @@ int mfix_solve_pp_g_dev(const mfix_pp_g_fields *fields, double normg,
                         int mixed, double *resg, int *ier,
                         mfix_leq_stats *stats, void *stream);
 
+/* ---- scalar transport equations, first-order upwinding, on the device ---- */
+/* Fields of CONV_DIF_PHI (model/conv_dif_phi.f:17) for one phase: DIMENSION_3
+ * device arrays in the storage contract above, and the 1-D geometry arrays
+ * of model/geometry_mod.f with Fortran index 1 first (I = 1..IMAX2 at
+ * [0..IMAX2-1], likewise J, and K = 1..KMAX2, one value when NO_K). */
+typedef struct mfix_phi_fields {
+    const double *dif;                        /* DIF, the diffusion coefficient */
+    const double *u, *v, *w;                  /* UF, VF, WF: the upwind side of each face */
+    const double *flux_e, *flux_n, *flux_t;   /* FLUX_E/N/T face mass fluxes */
+    const double *ayz, *axz, *axy;            /* face areas, model/geometry_mod.f:206-210 */
+    const double *odx_e, *ox, *fx_e, *fx_e_bar;   /* along I */
+    const double *ody_n, *fy_n, *fy_n_bar;        /* along J */
+    const double *odz_t, *fz_t, *fz_t_bar;        /* along K */
+    const int *flag;                          /* FLAG */
+} mfix_phi_fields;
+
+/* INIT_AB_M(A_M,B_M,IJKMAX2,M) + CONV_DIF_PHI(PHI,DIF,DISC=0,UF,VF,WF,FLUX_E,
+ * FLUX_N,FLUX_T,M,A_M,B_M) with DEF_COR off (model/init_ab_m.f:20,
+ * model/conv_dif_phi.f:17-80, :214-339): writes phase M's A_M(:,:,M) slice
+ * (d_a_m, 7 planes) and B_M(:,M) (d_b_m, zero).  No cut cells, no internal
+ * surfaces.  Returns once the kernel is queued on the stream. */
+int mfix_conv_dif_phi_dev(const mfix_phi_fields *fields, double *d_a_m,
+                          double *d_b_m, void *stream);
+
+/* SOURCE_PHI(S_P,S_C,EP,PHI,M,A_M,B_M) (model/source_phi.f:25-139) in place
+ * on phase M's slice, fluid rows only; no internal surfaces (IS_AT_* false
+ * away from walls).  DIL_EP_s is model/toleranc_mod.f:24's 1e-4 in MFIX.
+ * Queued on the stream. */
+int mfix_source_phi_dev(const double *d_s_p, const double *d_s_c,
+                        const double *d_ep, const double *d_phi,
+                        const int *d_flag, double dil_ep_s, double *d_a_m,
+                        double *d_b_m, void *stream);
+
+/* UNDER_RELAX_S(VAR,A_M,B_M,M,UR) (model/under_relax.f:20-81) in place on
+ * phase M's slice.  UR = 0 (a division by zero in the reference) is
+ * MFIX_LEQ_EINVAL.  Queued on the stream. */
+int mfix_under_relax_s_dev(const double *d_var, const int *d_flag, double ur,
+                           double *d_a_m, double *d_b_m, void *stream);
+
 /* ---- serving: continuous batching with HBM-budget admission ------------- */
 typedef struct mfix_leq_server mfix_leq_server;
 typedef struct mfix_leq_request {

@@ -62,6 +62,16 @@ class PPFields(C.Structure):
                                                       ("ijk_p_g", C.c_int)]
 
 
+# mfix_phi_fields order (include/mfix_leq.h): DIMENSION_3 fields, then the
+# 1-D geometry arrays along I, J and K
+PHI_FIELDS = ("dif", "u", "v", "w", "flux_e", "flux_n", "flux_t", "ayz", "axz", "axy")
+PHI_GEO = ("odx_e", "ox", "fx_e", "fx_e_bar", "ody_n", "fy_n", "fy_n_bar", "odz_t", "fz_t", "fz_t_bar")
+
+
+class PhiFields(C.Structure):
+    _fields_ = [(f, C.c_void_p) for f in PHI_FIELDS + PHI_GEO] + [("flag", C.c_void_p)]
+
+
 class BatchItem(C.Structure):
     _fields_ = [
         ("imax", C.c_int), ("jmax", C.c_int), ("kmax", C.c_int),
@@ -126,6 +136,10 @@ _SIGS = {
     "mfix_solve_pp_g_dev": (C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                                       C.c_int, C.c_char_p, C.c_double, C.c_char_p, C.c_int, C.c_void_p, C.c_void_p,
                                       C.c_void_p, C.c_void_p]),
+    "mfix_conv_dif_phi_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "mfix_source_phi_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
+                                      C.c_void_p, C.c_void_p, C.c_void_p]),
+    "mfix_under_relax_s_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]),
     "mfix_calc_resid_c": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
     "mfix_calc_resid_c_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
                                         C.c_void_p]),

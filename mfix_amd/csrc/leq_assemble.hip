@@ -22,8 +22,6 @@ namespace mfix {
 
 constexpr int kAsmBlock = 256;
 
-__device__ __forceinline__ bool pflow_at(int f) { return f == 10 || f == 11; }   // functions.inc:417-423
-
 struct AsmArgs {
     DGrid g;
     mfix_pp_g_fields f;

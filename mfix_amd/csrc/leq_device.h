@@ -85,6 +85,12 @@ __device__ __forceinline__ bool is_interior(const DGrid &g, int64_t c) {
     return g.do_k ? (ij && k > 0 && k < g.nk - 1) : ij;
 }
 
+// FLAG predicates, model/include/functions.inc:409-414 (FLUID_AT),
+// :417-423 (P_FLOW_AT) and :469-474 (WALL_AT).
+__device__ __forceinline__ bool fluid_at(int f) { return f == 1; }
+__device__ __forceinline__ bool wall_at(int f) { return f >= 100; }
+__device__ __forceinline__ bool pflow_at(int f) { return f == 10 || f == 11; }
+
 __device__ __forceinline__ Nb neighbours(const DGrid &g, int64_t c) {
     int i, j, k;
     cell_ijk(g, c, i, j, k);

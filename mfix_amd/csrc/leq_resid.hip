@@ -31,11 +31,6 @@ struct ResidArgs {
     mfix_leq_resid *out;
 };
 
-// FLAG predicates, model/include/functions.inc:409-414 (FLUID_AT) and
-// :469-474 (WALL_AT).
-__device__ __forceinline__ bool fluid_at(int f) { return f == 1; }
-__device__ __forceinline__ bool wall_at(int f) { return f >= 100; }
-
 // The larger candidate (value, 0-based cell); equal values keep the first
 // cell, as the reference's scan with a strict '>' does.
 __device__ __forceinline__ void better(double &v, double &i, double ov, double oi) {

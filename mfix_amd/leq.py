@@ -378,3 +378,41 @@ def solve_pp_g_dev(fields: dict, flag: "DeviceArray", odt: float, ijk_p_g: int, 
                                       sweep.encode(), tol, pc.encode(), int(mixed), C.byref(resg), C.byref(ier),
                                       C.byref(st), None), "mfix_solve_pp_g_dev")
     return {"resg": resg.value, "ier": ier.value, "stats": st.as_dict()}
+
+
+# ---- scalar transport equations on the device (first-order upwinding) ------
+# (INIT_AB_M + CONV_DIF_PHI, SOURCE_PHI, UNDER_RELAX_S: model/conv_dif_phi.f,
+#  model/source_phi.f, model/under_relax.f)
+
+def conv_dif_phi_dev(fields: dict, flag: "DeviceArray", a: "DeviceArray", b: "DeviceArray", *,
+                     sync: bool = True) -> None:
+    """INIT_AB_M + CONV_DIF_PHI (DISC = 0) into a (7 planes) and b from resident
+    fields (DeviceArrays keyed by the names of mfix_phi_fields)."""
+    f = _lib.PhiFields()
+    for k in _lib.PHI_FIELDS + _lib.PHI_GEO:
+        setattr(f, k, fields[k].ptr)
+    f.flag = flag.ptr
+    L = _lib_()
+    check(L.mfix_conv_dif_phi_dev(C.byref(f), a.ptr, b.ptr, None), "mfix_conv_dif_phi_dev")
+    if sync:
+        check(L.mfix_leq_sync(), "mfix_leq_sync")
+
+
+def source_phi_dev(s_p: "DeviceArray", s_c: "DeviceArray", ep: "DeviceArray", phi: "DeviceArray",
+                   flag: "DeviceArray", a: "DeviceArray", b: "DeviceArray", *, dil_ep_s: float = 1.0e-4,
+                   sync: bool = True) -> None:
+    """SOURCE_PHI in place on (a, b)."""
+    L = _lib_()
+    check(L.mfix_source_phi_dev(s_p.ptr, s_c.ptr, ep.ptr, phi.ptr, flag.ptr, dil_ep_s, a.ptr, b.ptr, None),
+          "mfix_source_phi_dev")
+    if sync:
+        check(L.mfix_leq_sync(), "mfix_leq_sync")
+
+
+def under_relax_s_dev(var: "DeviceArray", flag: "DeviceArray", ur: float, a: "DeviceArray", b: "DeviceArray", *,
+                      sync: bool = True) -> None:
+    """UNDER_RELAX_S in place on (a, b)."""
+    L = _lib_()
+    check(L.mfix_under_relax_s_dev(var.ptr, flag.ptr, ur, a.ptr, b.ptr, None), "mfix_under_relax_s_dev")
+    if sync:
+        check(L.mfix_leq_sync(), "mfix_leq_sync")

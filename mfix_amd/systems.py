@@ -105,3 +105,40 @@ def pp_g_fields(imax: int, jmax: int, kmax: int, seed: int) -> tuple[dict, np.nd
     flag[1:-1, 0, 1:-1] = 20
     flag[1:-1, ni - 1, 1:-1] = 11
     return f, flag.reshape(-1)
+
+
+def phi_fields(imax: int, jmax: int, kmax: int, seed: int, odt: float = 100.0) -> tuple[dict, np.ndarray]:
+    """Seeded fields of a gas-phase scalar transport equation (the inputs of
+    CONV_DIF_PHI, SOURCE_PHI and UNDER_RELAX_S) on a box of uniform 1 cm
+    cells: DIF ~ 1e-5, velocities in [-1,1] with mass fluxes ROP*U*A, the
+    accumulation source S_P = ROP_o*VOL*ODT and S_C = S_P*PHI_old, EP in
+    [0.4, 1]; FLAG as in pp_g_fields (ghost walls, inflow I- face, pressure
+    outflow I+ face, fluid inside).  The 1-D geometry is uniform (ODX_E =
+    1/DX, FX_E = 1/2, OX = 1)."""
+    ni, nj, nk = imax + 2, jmax + 2, kmax + 2
+    n3 = ni * nj * nk
+    rng = np.random.default_rng(seed)
+    dx = 0.01
+    u, v, w = (rng.uniform(-1, 1, n3) for _ in range(3))
+    rop = rng.uniform(0.9, 1.1, n3)
+    area = np.full(n3, dx * dx)
+    s_p = rop * dx ** 3 * odt
+    phi_old = rng.uniform(0.0, 1.0, n3)
+    f = {
+        "dif": rng.uniform(0.5e-5, 2e-5, n3), "u": u, "v": v, "w": w,
+        "flux_e": rop * u * area, "flux_n": rop * v * area, "flux_t": rop * w * area,
+        "ayz": area, "axz": area.copy(), "axy": area.copy(),
+        "s_p": s_p, "s_c": s_p * phi_old, "ep": rng.uniform(0.4, 1.0, n3), "phi": phi_old.copy(),
+    }
+    for name, n in (("x", ni), ("y", nj), ("z", nk)):
+        o = np.full(n, 1.0 / dx)
+        fr = np.full(n, 0.5)
+        f.update({{"x": "odx_e", "y": "ody_n", "z": "odz_t"}[name]: o,
+                  {"x": "fx_e", "y": "fy_n", "z": "fz_t"}[name]: fr,
+                  {"x": "fx_e_bar", "y": "fy_n_bar", "z": "fz_t_bar"}[name]: fr.copy()})
+    f["ox"] = np.ones(ni)
+    flag = np.full((nk, ni, nj), 100, dtype=np.int32)
+    flag[1:-1, 1:-1, 1:-1] = 1
+    flag[1:-1, 0, 1:-1] = 20
+    flag[1:-1, ni - 1, 1:-1] = 11
+    return f, flag.reshape(-1)

@@ -59,6 +59,20 @@ class OPPFields(C.Structure):
                                                 ("ijk_p_g", C.c_int)]
 
 
+# scalar transport (og_phi_fields, ref_harness op 11): DIMENSION_3 fields,
+# then the 1-D geometry arrays along I, J and K (Fortran index 1.. at [0..])
+PHI_FIELDS = ("dif", "u", "v", "w", "flux_e", "flux_n", "flux_t", "ayz", "axz", "axy")
+PHI_GEO_I = ("odx_e", "ox", "fx_e", "fx_e_bar")
+PHI_GEO_J = ("ody_n", "fy_n", "fy_n_bar")
+PHI_GEO_K = ("odz_t", "fz_t", "fz_t_bar")
+PHI_SOURCES = ("s_p", "s_c", "ep", "phi")
+
+
+class OPhiFields(C.Structure):
+    _fields_ = [(f, _dp) for f in PHI_FIELDS + PHI_GEO_I + PHI_GEO_J + PHI_GEO_K] + [
+        ("flag", C.POINTER(C.c_int))]
+
+
 def build_oracle(force: bool = False) -> Path:
     if force or not LIB.exists() or LIB.stat().st_mtime < (HERE / "leq_oracle.c").stat().st_mtime:
         subprocess.run(["make", "-s", "-C", str(HERE), "oracle"], check=True)
@@ -100,6 +114,13 @@ def lib() -> C.CDLL:
                                         _dp, C.c_double, C.c_double, C.c_int, C.POINTER(OResid)]
         l.oracle_assemble_pp_g.argtypes = [C.POINTER(OGrid), C.POINTER(OPPFields), _dp, _dp, _dp]
         l.oracle_assemble_pp_g.restype = C.c_int
+        l.oracle_conv_dif_phi0.argtypes = [C.POINTER(OGrid), C.POINTER(OPhiFields), _dp, _dp]
+        l.oracle_conv_dif_phi0.restype = None
+        l.oracle_source_phi.argtypes = [C.POINTER(OGrid), _dp, _dp, _dp, _dp, C.POINTER(C.c_int), C.c_double,
+                                        _dp, _dp]
+        l.oracle_source_phi.restype = None
+        l.oracle_under_relax_s.argtypes = [C.POINTER(OGrid), _dp, C.POINTER(C.c_int), C.c_double, _dp, _dp]
+        l.oracle_under_relax_s.restype = None
         _lib = l
     return _lib
 
@@ -203,16 +224,58 @@ def assemble_pp_g(g, fields: dict, flag, odt, ijk_p_g=0):
     return a, b, bm, bad
 
 
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _iptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def conv_dif_phi(g, fields: dict, flag):
+    """Oracle INIT_AB_M + CONV_DIF_PHI (first-order upwinding, DISC = 0);
+    fields holds PHI_FIELDS and the 1-D geometry arrays.  Returns (A_M, B_M)."""
+    n3 = n3_of(g)
+    keep = {k: _f64(fields[k]) for k in PHI_FIELDS + PHI_GEO_I + PHI_GEO_J + PHI_GEO_K}
+    fl = np.ascontiguousarray(flag, dtype=np.int32)
+    F = OPhiFields(*[_p(keep[k]) for k in PHI_FIELDS + PHI_GEO_I + PHI_GEO_J + PHI_GEO_K], _iptr(fl))
+    a = np.zeros(7 * n3)
+    b = np.zeros(n3)
+    lib().oracle_conv_dif_phi0(C.byref(g), C.byref(F), _p(a), _p(b))
+    return a, b
+
+
+def source_phi(g, a, b, s_p, s_c, ep, phi, flag, dil_ep_s=1.0e-4):
+    """Oracle SOURCE_PHI on copies of (A_M, B_M); returns the new pair."""
+    a = _f64(a).copy(); b = _f64(b).copy()
+    fl = np.ascontiguousarray(flag, dtype=np.int32)
+    lib().oracle_source_phi(C.byref(g), _p(_f64(s_p)), _p(_f64(s_c)), _p(_f64(ep)), _p(_f64(phi)), _iptr(fl),
+                            dil_ep_s, _p(a), _p(b))
+    return a, b
+
+
+def under_relax_s(g, a, b, var, flag, ur):
+    """Oracle UNDER_RELAX_S on copies of (A_M, B_M); returns the new pair."""
+    a = _f64(a).copy(); b = _f64(b).copy()
+    fl = np.ascontiguousarray(flag, dtype=np.int32)
+    lib().oracle_under_relax_s(C.byref(g), _p(_f64(var)), _iptr(fl), ur, _p(a), _p(b))
+    return a, b
+
+
 # ---- driver of the compiled reference (ref_harness.f90) --------------------
 OPS = {"bicgs": 1, "matvec": 2, "msolve": 3, "msolve1": 4, "dot": 5, "dot2": 6,
-       "resid_c": 7, "resid_s": 8, "resid_pp": 9, "assemble_pp": 10}
+       "resid_c": 7, "resid_s": 8, "resid_pp": 9, "assemble_pp": 10, "phi": 11}
 
 
 def run_reference(op, g: OGrid, a, b, x, pc="DIAG", cmethod="RSRS", tol=1e-10, itmax=500, seed=12345,
-                  workdir: Path | None = None, resid: dict | None = None, pp: dict | None = None):
+                  workdir: Path | None = None, resid: dict | None = None, pp: dict | None = None,
+                  phi: dict | None = None):
     """Run one reference routine through oracle/_ref/leq_ref_harness.  The
     residual ops take resid = {flag, debug, m, tol, norm, ep_s}, the
-    pressure-correction assembly op pp = {flag, odt, ijk_p_g, fields}."""
+    pressure-correction assembly op pp = {flag, odt, ijk_p_g, fields}, the
+    scalar transport op phi = {flag, stage, ur, fields} (fields: PHI_FIELDS,
+    PHI_SOURCES and the 1-D geometry arrays; stage 1 CONV_DIF_PHI, 2
+    SOURCE_PHI, 3 UNDER_RELAX_S on the a, b given)."""
     if not REF_BIN.exists():
         raise FileNotFoundError(REF_BIN)
     n3 = n3_of(g)
@@ -235,6 +298,13 @@ def run_reference(op, g: OGrid, a, b, x, pc="DIAG", cmethod="RSRS", tol=1e-10, i
             f.write(np.ascontiguousarray(pp["flag"], dtype=np.int32).tobytes())
             f.write(np.stack([np.ascontiguousarray(pp["fields"][k], dtype=np.float64) for k in PP_FIELDS],
                              axis=1).T.copy().tobytes())
+        elif OPS[op] == 11:
+            fd = phi["fields"]
+            f.write(np.array([int(phi["stage"])], dtype=np.int32).tobytes())
+            f.write(np.float64(phi.get("ur", 1.0)).tobytes())
+            f.write(np.ascontiguousarray(phi["flag"], dtype=np.int32).tobytes())
+            for keys in (PHI_FIELDS + PHI_SOURCES, PHI_GEO_I, PHI_GEO_J, PHI_GEO_K):
+                f.write(np.stack([_f64(fd[k]) for k in keys]).tobytes())
         elif OPS[op] >= 7:
             f.write(np.array([int(resid["debug"]), int(resid["m"])], dtype=np.int32).tobytes())
             f.write(np.array([resid.get("tol", 0.0), resid.get("norm", 1.0)], dtype=np.float64).tobytes())

@@ -811,3 +811,194 @@ int oracle_assemble_pp_g(const og_grid *g, const og_pp_fields *F, double *a, dou
     }
     return bad;
 }
+
+/* ---- scalar transport, first-order upwinding ----
+ *   INIT_AB_M            model/init_ab_m.f:20-78
+ *   CONV_DIF_PHI         model/conv_dif_phi.f:17-80 with DISC = 0 and DEF_COR
+ *                        off: CONV_DIF_PHI0 (:214-339, the reference's own
+ *                        scatter order) with GET_PHICELL_DIFF_TERMS (:87-198,
+ *                        no cut cells) and DIF_PHI_IS (:722-836), which adds
+ *                        nothing without internal surfaces
+ *   SOURCE_PHI           model/source_phi.f:25-139 (no point sources)
+ *   UNDER_RELAX_S        model/under_relax.f:20-81
+ * The 1-D geometry arrays hold Fortran index I = 1..IMAX2 at [I-1]. */
+typedef struct {
+    const double *dif, *u, *v, *w, *flux_e, *flux_n, *flux_t, *ayz, *axz, *axy;
+    const double *odx_e, *ox, *fx_e, *fx_e_bar, *ody_n, *fy_n, *fy_n_bar, *odz_t, *fz_t, *fz_t_bar;
+    const int *flag;
+} og_phi_fields;
+
+/* AVG_X_H / AVG_Y_H / AVG_Z_H, model/include/fun_avg.inc:93-129 */
+static inline double og_avg_h(double m, double p, double f, double fbar) {
+    const double den = f * m + fbar * p;
+    return m * p / (den > 1.0e-15 ? den : 1.0e-15);   /* MAX(SMALL_NUMBER, .) */
+}
+
+/* GET_PHICELL_DIFF_TERMS for fluid cell (i,j,k): D = fe, fw, fn, fs, ft, fb */
+static void og_phi_diff(const og_dims *d, const og_phi_fields *F, int i, int j, int k, const int64_t mp[6],
+                        const int64_t nb[6], double D[6]) {
+    const og_grid *g = d->g;
+    const int64_t c = IJK(d, i, j, k);
+    const int im = nb_minus(i, d->ni, g->cyclic_x), jm = nb_minus(j, d->nj, g->cyclic_y);
+    const int km = d->do_k ? nb_minus(k, d->nk, g->cyclic_z) : k;
+    const double *dif = F->dif;
+    const double c_ae = F->odx_e[i - 1] * F->ayz[c];
+    const double c_aw = F->odx_e[im - 1] * F->ayz[mp[0]];
+    const double c_an = F->ody_n[j - 1] * F->axz[c];
+    const double c_as = F->ody_n[jm - 1] * F->axz[mp[2]];
+    const double c_at = F->ox[i - 1] * F->odz_t[k - 1] * F->axy[c];
+    const double c_ab = F->ox[i - 1] * F->odz_t[km - 1] * F->axy[mp[4]];
+    /* nb = E, W, S, N, B, T */
+    D[0] = og_avg_h(dif[c], dif[nb[0]], F->fx_e[i - 1], F->fx_e_bar[i - 1]) * c_ae;
+    D[1] = og_avg_h(dif[nb[1]], dif[c], F->fx_e[im - 1], F->fx_e_bar[im - 1]) * c_aw;
+    D[2] = og_avg_h(dif[c], dif[nb[3]], F->fy_n[j - 1], F->fy_n_bar[j - 1]) * c_an;
+    D[3] = og_avg_h(dif[nb[2]], dif[c], F->fy_n[jm - 1], F->fy_n_bar[jm - 1]) * c_as;
+    D[4] = D[5] = 0.0;
+    if (d->do_k) {
+        D[4] = og_avg_h(dif[c], dif[nb[5]], F->fz_t[k - 1], F->fz_t_bar[k - 1]) * c_at;
+        D[5] = og_avg_h(dif[nb[4]], dif[c], F->fz_t[km - 1], F->fz_t_bar[km - 1]) * c_ab;
+    }
+}
+
+void oracle_conv_dif_phi0(const og_grid *g, const og_phi_fields *F, double *a, double *b) {
+    og_dims d = dims_of(g);
+    const int64_t n3 = d.n3;
+    const int *flag = F->flag;
+    for (int64_t c = 0; c < n3; ++c) {
+        for (int l = -3; l <= 3; ++l) A(l, c) = 0.0;
+        A(0, c) = -1.0;
+        b[c] = 0.0;
+    }
+    for (int k = 1; k <= d.nk; ++k)
+        for (int i = 1; i <= d.ni; ++i)
+            for (int j = 1; j <= d.nj; ++j) {
+                const int64_t c = IJK(&d, i, j, k);
+                if (!og_fluid(flag[c])) continue;
+                int64_t mp[6], nb[6];
+                og_index1a(&d, flag, i, j, k, mp, nb);
+                double D[6];
+                og_phi_diff(&d, F, i, j, k, mp, nb, D);
+                /* mp = IM, IP, JM, JP, KM, KP */
+                if (F->u[c] >= 0.0) {
+                    A(1, c) = D[0];
+                    A(-1, mp[1]) = D[0] + F->flux_e[c];
+                } else {
+                    A(1, c) = D[0] - F->flux_e[c];
+                    A(-1, mp[1]) = D[0];
+                }
+                if (!og_fluid(flag[mp[0]])) A(-1, c) = F->u[mp[0]] >= 0.0 ? D[1] + F->flux_e[mp[0]] : D[1];
+                if (F->v[c] >= 0.0) {
+                    A(2, c) = D[2];
+                    A(-2, mp[3]) = D[2] + F->flux_n[c];
+                } else {
+                    A(2, c) = D[2] - F->flux_n[c];
+                    A(-2, mp[3]) = D[2];
+                }
+                if (!og_fluid(flag[mp[2]])) A(-2, c) = F->v[mp[2]] >= 0.0 ? D[3] + F->flux_n[mp[2]] : D[3];
+                if (d.do_k) {
+                    if (F->w[c] >= 0.0) {
+                        A(3, c) = D[4];
+                        A(-3, mp[5]) = D[4] + F->flux_t[c];
+                    } else {
+                        A(3, c) = D[4] - F->flux_t[c];
+                        A(-3, mp[5]) = D[4];
+                    }
+                    if (!og_fluid(flag[mp[4]])) A(-3, c) = F->w[mp[4]] >= 0.0 ? D[5] + F->flux_t[mp[4]] : D[5];
+                }
+            }
+}
+
+/* FLAG_E/N/T as SET_FLAGS leaves them without internal surfaces
+ * (model/set_flags.f:403-470, storage order; UNDEFINED_I from
+ * model/param1_mod.f:19; CYCLIC_AT = FLAG 106 or 107, functions.inc:524-530). */
+static void og_face_flags(const og_dims *d, const int *flag, int *fe, int *fn, int *ft) {
+    const int und = 987654321;
+    for (int64_t c = 0; c < d->n3; ++c) fe[c] = fn[c] = ft[c] = und;
+    for (int k = 1; k <= d->nk; ++k)
+        for (int i = 1; i <= d->ni; ++i)
+            for (int j = 1; j <= d->nj; ++j) {
+                const int64_t c = IJK(d, i, j, k);
+                int64_t mp[6], nb[6];
+                og_index1a(d, flag, i, j, k, mp, nb);
+                if (og_wall(flag[c])) {
+                    fe[c] = fn[c] = ft[c] = 0;
+                    fe[mp[0]] = 0;
+                    fn[mp[2]] = 0;
+                    ft[mp[4]] = 0;
+                    if (flag[c] == 106 || flag[c] == 107) {
+                        const int jin = j != 1 && j != d->nj, iin = i != 1 && i != d->ni;
+                        const int kin = !d->do_k || (k != 1 && k != d->nk);
+                        if (i == d->ni && jin && kin && !og_wall(flag[mp[0]])) fe[mp[0]] = 2000;
+                        if (j == d->nj && iin && kin && !og_wall(flag[mp[2]])) fn[mp[2]] = 2000;
+                        if (k == d->nk && jin && iin && !og_wall(flag[mp[4]])) ft[mp[4]] = 2000;
+                    }
+                } else if (og_fluid(flag[c])) {
+                    if (!og_wall(flag[mp[0]]) && fe[mp[0]] == und) fe[mp[0]] = 2000 + flag[mp[0]];
+                    if (!og_wall(flag[mp[2]]) && fn[mp[2]] == und) fn[mp[2]] = 2000 + flag[mp[2]];
+                    if (!og_wall(flag[mp[4]]) && ft[mp[4]] == und) ft[mp[4]] = 2000 + flag[mp[4]];
+                    if (!og_wall(flag[mp[1]]) && fe[c] == und) fe[c] = 2000 + flag[mp[1]];
+                    if (!og_wall(flag[mp[3]]) && fn[c] == und) fn[c] = 2000 + flag[mp[3]];
+                    if (!og_wall(flag[mp[5]]) && ft[c] == und) ft[c] = 2000 + flag[mp[5]];
+                }
+            }
+}
+
+/* SOURCE_PHI on the fluid rows, in place; IS_AT_E/N/T (functions.inc:760-781)
+ * read the face flags above. */
+void oracle_source_phi(const og_grid *g, const double *s_p, const double *s_c, const double *ep, const double *phi,
+                       const int *flag, double dil_ep_s, double *a, double *b) {
+    og_dims d = dims_of(g);
+    const int64_t n3 = d.n3;
+    int *fe = malloc(sizeof(int) * 3 * n3);
+    int *fn = fe + n3, *ft = fe + 2 * n3;
+    og_face_flags(&d, flag, fe, fn, ft);
+    for (int k = 1; k <= d.nk; ++k)
+        for (int i = 1; i <= d.ni; ++i)
+            for (int j = 1; j <= d.nj; ++j) {
+                const int64_t c = IJK(&d, i, j, k);
+                if (!og_fluid(flag[c])) continue;
+                int64_t mp[6], nb[6];
+                og_index1a(&d, flag, i, j, k, mp, nb);
+                if (ep[c] <= dil_ep_s) {
+                    for (int l = -3; l <= 3; ++l) A(l, c) = 0.0;
+                    A(0, c) = -1.0;
+                    b[c] = 0.0;
+                    /* nb = E, W, S, N, B, T; mp = IM, IP, JM, JP, KM, KP */
+                    if (ep[nb[1]] > dil_ep_s && !(fe[mp[0]] < 2000)) A(-1, c) = 1.0;
+                    if (ep[nb[0]] > dil_ep_s && !(fe[c] < 2000)) A(1, c) = 1.0;
+                    if (ep[nb[2]] > dil_ep_s && !(fn[mp[2]] < 2000)) A(-2, c) = 1.0;
+                    if (ep[nb[3]] > dil_ep_s && !(fn[c] < 2000)) A(2, c) = 1.0;
+                    if (d.do_k) {
+                        if (ep[nb[4]] > dil_ep_s && !(ft[mp[4]] < 2000)) A(-3, c) = 1.0;
+                        if (ep[nb[5]] > dil_ep_s && !(ft[c] < 2000)) A(3, c) = 1.0;
+                    }
+                    if ((A(-1, c) + A(1, c) + A(-2, c) + A(2, c) + A(-3, c) + A(3, c)) == 0.0)
+                        b[c] = -phi[c];
+                    else
+                        A(0, c) = -(A(1, c) + A(-1, c) + A(2, c) + A(-2, c) + A(3, c) + A(-3, c));
+                } else {
+                    A(0, c) = -(A(1, c) + A(-1, c) + A(2, c) + A(-2, c) + A(3, c) + A(-3, c) + s_p[c]);
+                    if (b[c] < s_c[c] || phi[c] == 0.0) {
+                        b[c] = -s_c[c] + b[c];
+                    } else {
+                        A(0, c) = A(0, c) - b[c] / phi[c];
+                        b[c] = -s_c[c];
+                    }
+                }
+            }
+    free(fe);
+}
+
+void oracle_under_relax_s(const og_grid *g, const double *var, const int *flag, double ur, double *a, double *b) {
+    og_dims d = dims_of(g);
+    const int64_t n3 = d.n3;
+    const double f1 = 1.0 / ur, f2 = f1 - 1.0;
+    for (int64_t c = 0; c < n3; ++c) {
+        if (!og_fluid(flag[c])) continue;
+        const double ap = A(0, c);
+        if (ap != -1.0) {
+            A(0, c) = ap * f1;
+            b[c] = b[c] + ap * var[c] * f2;
+        }
+    }
+}

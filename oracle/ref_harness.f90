@@ -33,6 +33,15 @@
 ! int32 and 16 real64 arrays of N3: ROP_gE, ROP_gN, ROP_gT, AYZ, AXZ, AXY,
 ! VOL, ROP_g, ROP_go, U_g, V_g, W_g, SUM_R_g, D_E(:,0), D_N(:,0), D_T(:,0);
 ! it returns the assembled A_M/B_M in A/B and B_MMAX in X.
+! Op 11 (scalar transport, phase 0) adds, after X: int32 stage, real64 UR,
+! FLAG(N3) int32, 14 real64 arrays of N3 (DIF, U, V, W, FLUX_E, FLUX_N,
+! FLUX_T, AYZ, AXZ, AXY, S_P, S_C, EP, PHI), then ODX_E, OX, FX_E, FX_E_bar
+! (IMAX2 each, from I = 1), ODY_N, FY_N, FY_N_bar (JMAX2), ODZ_T, FZ_T,
+! FZ_T_bar (KMAX2).  Stage 1: INIT_AB_M + CONV_DIF_PHI with DISC = 0
+! (model/init_ab_m.f:20, model/conv_dif_phi.f:17); stage 2: SOURCE_PHI
+! (model/source_phi.f:25) on the A/B read in; stage 3: UNDER_RELAX_S
+! (model/under_relax.f:20) with VAR = PHI on the A/B read in.  FLAG_E/N/T
+! follow SET_FLAGS (model/set_flags.f:400-470) without internal surfaces.
 program ref_harness
   use param, only: dimension_3
   use compar
@@ -53,7 +62,14 @@ program ref_harness
   use physprop, only: mmax, ro_g0
   use bc, only: ijk_p_g
   use param1, only: undefined_i
+  use cutcell, only: cut_treatment_at, cut_cell_at
+  use run, only: def_cor
+  use is, only: is_defined
+  use functions, only: wall_at, fluid_at, im_of, jm_of, km_of, ip_of, jp_of, kp_of
   implicit none
+  integer :: shdr(1)
+  double precision :: sur
+  double precision, allocatable :: sf(:,:), g1i(:,:), g1j(:,:), g1k(:,:)
   integer :: phdr(1)
   double precision :: podt
   double precision, allocatable :: pf(:,:), bmm(:,:)
@@ -195,6 +211,16 @@ program ref_harness
      read(10) flagin
      read(10) pf
   end if
+  if (op == 11) then
+     read(10) shdr
+     read(10) sur
+     allocate(flagin(n3), sf(n3, 14), g1i(ni, 4), g1j(nj, 3), g1k(nk, 3))
+     read(10) flagin
+     read(10) sf
+     read(10) g1i
+     read(10) g1j
+     read(10) g1k
+  end if
   if (op >= 7 .and. op <= 9) then
      read(10) rhdr
      read(10) rtol, rnorm
@@ -205,7 +231,7 @@ program ref_harness
   close(10)
 
   if (op >= 7) then
-     if (op == 10) then
+     if (op == 10 .or. op == 11) then
         rhdr = 0
         rtol = 0d0; rnorm = 1d0
         allocate(epsin(n3))
@@ -331,6 +357,73 @@ program ref_harness
      amat = a3(:, :, 0)
      bvec = b2(:, 0)
      yvec = bmm(:, 0)
+  case (11)
+     allocate(ayz(n3), axz(n3), axy(n3), flag_e(n3), flag_n(n3), flag_t(n3))
+     allocate(cut_treatment_at(n3), cut_cell_at(n3))
+     allocate(odx_e(0:ni), ox(0:ni), fx_e(0:ni), fx_e_bar(0:ni))
+     allocate(ody_n(0:nj), fy_n(0:nj), fy_n_bar(0:nj), odz_t(0:nk), fz_t(0:nk), fz_t_bar(0:nk))
+     ayz = sf(:, 8); axz = sf(:, 9); axy = sf(:, 10)
+     odx_e = 0d0; ox = 0d0; fx_e = 0d0; fx_e_bar = 0d0
+     ody_n = 0d0; fy_n = 0d0; fy_n_bar = 0d0; odz_t = 0d0; fz_t = 0d0; fz_t_bar = 0d0
+     odx_e(1:ni) = g1i(:, 1); ox(1:ni) = g1i(:, 2); fx_e(1:ni) = g1i(:, 3); fx_e_bar(1:ni) = g1i(:, 4)
+     ody_n(1:nj) = g1j(:, 1); fy_n(1:nj) = g1j(:, 2); fy_n_bar(1:nj) = g1j(:, 3)
+     odz_t(1:nk) = g1k(:, 1); fz_t(1:nk) = g1k(:, 2); fz_t_bar(1:nk) = g1k(:, 3)
+     cut_treatment_at = .false.; cut_cell_at = .false.
+     def_cor = .false.
+     is_defined = .false.
+     ! FLAG_E/N/T as SET_FLAGS leaves them with no internal surface
+     flag_e = undefined_i; flag_n = undefined_i; flag_t = undefined_i
+     do ijk = 1, n3
+        i = i_of(ijk); j = j_of(ijk); k = k_of(ijk)
+        if (wall_at(ijk)) then
+           flag_e(ijk) = 0; flag_n(ijk) = 0; flag_t(ijk) = 0
+           flag_e(im_of(ijk)) = 0; flag_n(jm_of(ijk)) = 0; flag_t(km_of(ijk)) = 0
+           if (flag(ijk) == 106 .or. flag(ijk) == 107) then
+              if (i == imax2 .and. j /= 1 .and. j /= jmax2) then
+                 if (no_k) then
+                    if (.not. wall_at(im_of(ijk))) flag_e(im_of(ijk)) = 2000
+                 else if (k /= 1 .and. k /= kmax2) then
+                    if (.not. wall_at(im_of(ijk))) flag_e(im_of(ijk)) = 2000
+                 end if
+              end if
+              if (j == jmax2 .and. i /= 1 .and. i /= imax2) then
+                 if (no_k) then
+                    if (.not. wall_at(jm_of(ijk))) flag_n(jm_of(ijk)) = 2000
+                 else if (k /= 1 .and. k /= kmax2) then
+                    if (.not. wall_at(jm_of(ijk))) flag_n(jm_of(ijk)) = 2000
+                 end if
+              end if
+              if (k == kmax2 .and. j /= 1 .and. j /= jmax2 .and. i /= 1 .and. i /= imax2) then
+                 if (.not. wall_at(km_of(ijk))) flag_t(km_of(ijk)) = 2000
+              end if
+           end if
+        else if (fluid_at(ijk)) then
+           if (.not. wall_at(im_of(ijk)) .and. flag_e(im_of(ijk)) == undefined_i) &
+              flag_e(im_of(ijk)) = 2000 + flag(im_of(ijk))
+           if (.not. wall_at(jm_of(ijk)) .and. flag_n(jm_of(ijk)) == undefined_i) &
+              flag_n(jm_of(ijk)) = 2000 + flag(jm_of(ijk))
+           if (.not. wall_at(km_of(ijk)) .and. flag_t(km_of(ijk)) == undefined_i) &
+              flag_t(km_of(ijk)) = 2000 + flag(km_of(ijk))
+           if (.not. wall_at(ip_of(ijk)) .and. flag_e(ijk) == undefined_i) &
+              flag_e(ijk) = 2000 + flag(ip_of(ijk))
+           if (.not. wall_at(jp_of(ijk)) .and. flag_n(ijk) == undefined_i) &
+              flag_n(ijk) = 2000 + flag(jp_of(ijk))
+           if (.not. wall_at(kp_of(ijk)) .and. flag_t(ijk) == undefined_i) &
+              flag_t(ijk) = 2000 + flag(kp_of(ijk))
+        end if
+     end do
+     if (shdr(1) == 1) then
+        call init_ab_m(a3, b2, n3, 0)
+        call conv_dif_phi(sf(:, 14), sf(:, 1), 0, sf(:, 2), sf(:, 3), sf(:, 4), sf(:, 5), sf(:, 6), &
+                          sf(:, 7), 0, a3, b2)
+     else if (shdr(1) == 2) then
+        call source_phi(sf(:, 11), sf(:, 12), sf(:, 13), sf(:, 14), 0, a3, b2)
+     else
+        call under_relax_s(sf(:, 14), a3, b2, 0, sur)
+     end if
+     amat = a3(:, :, 0)
+     bvec = b2(:, 0)
+     yvec = 0d0
   case (7, 8, 9)
      rres = 0d0; rmax = 0d0; ijkr = 0
      if (op == 7) then
