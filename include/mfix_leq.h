@@ -296,9 +296,10 @@ int mfix_conv_dif_phi_dev(const mfix_phi_fields *fields, double *d_a_m,
                           double *d_b_m, void *stream);
 
 /* SOURCE_PHI(S_P,S_C,EP,PHI,M,A_M,B_M) (model/source_phi.f:25-139) in place
- * on phase M's slice, fluid rows only; no internal surfaces (IS_AT_* false
- * away from walls).  DIL_EP_s is model/toleranc_mod.f:24's 1e-4 in MFIX.
- * Queued on the stream. */
+ * on phase M's slice, fluid rows only; no internal surfaces, and a cyclic
+ * axis's ghost layers carry FLAG 106/107 as SET_FLAGS gives them (IS_AT_*
+ * then never changes the result).  DIL_EP_s is model/toleranc_mod.f:24's
+ * 1e-4 in MFIX.  Queued on the stream. */
 int mfix_source_phi_dev(const double *d_s_p, const double *d_s_c,
                         const double *d_ep, const double *d_phi,
                         const int *d_flag, double dil_ep_s, double *d_a_m,

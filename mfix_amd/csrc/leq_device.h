@@ -333,6 +333,28 @@ __device__ __forceinline__ bool xcd_valid(int64_t nblocks) {
     return r < (x < full ? per : per - 1);
 }
 
+// Plane-marching order for the one-pass assembly kernels, which read many
+// fields at K-1/K+1 and I-1/I+1.  The in-plane indices [0, c2) are cut into
+// chunks of bs cells; kMarchGroup consecutive chunks form a group, and a
+// group's blocks run plane after plane (K fastest).  The K-1 plane a block
+// reads was loaded by the block kMarchGroup logical blocks earlier, and the
+// I-1/I+1 rows lie in the same group except at its two edges, so both are
+// L2 hits; with the XCD remap above each XCD marches its own groups.
+constexpr int kMarchGroup = 8;
+__host__ __device__ inline int64_t march_blocks(int64_t c2, int nk, int bs) {
+    const int64_t groups = ((c2 + bs - 1) / bs + kMarchGroup - 1) / kMarchGroup;
+    return groups * kMarchGroup * nk;
+}
+// Cell of thread t of logical block L, or -1 past the plane's last cell.
+// Logical blocks number fewer than 2^31 (DIMENSION_3 < 2^31).
+__device__ __forceinline__ int64_t march_cell(const DGrid &g, int64_t L, int bs, int t) {
+    const uint32_t per_group = (uint32_t)kMarchGroup * (uint32_t)g.nk;
+    const uint32_t l = (uint32_t)L, grp = l / per_group, r = l - grp * per_group;
+    const uint32_t k = r / kMarchGroup, q = r - k * kMarchGroup;
+    const int64_t m = ((int64_t)grp * kMarchGroup + q) * bs + t;
+    return m < g.c2 ? (int64_t)k * g.c2 + m : -1;
+}
+
 // ---- deterministic reductions (fixed tree, wave64 shuffles) ----------------
 // block_sum leaves the block totals in thread 0; lds holds NV * (BS/64).
 template <int BS, int NV>

@@ -53,10 +53,11 @@ __device__ __forceinline__ double avg_h(double m, double p, double f, double fba
     return m * p / fmax(kSmallNumber, f * m + fbar * p);
 }
 
-__global__ void __launch_bounds__(kPhiBlock) k_conv_dif_phi(PhiArgs R) {
+__global__ void __launch_bounds__(kPhiBlock) k_conv_dif_phi(PhiArgs R, int64_t nblk) {
     const DGrid &g = R.g;
-    const int64_t c = (int64_t)blockIdx.x * kPhiBlock + threadIdx.x;
-    if (c >= g.n3) return;
+    if (!xcd_valid(nblk)) return;
+    const int64_t c = march_cell(g, xcd_block(nblk), kPhiBlock, threadIdx.x);
+    if (c < 0) return;
     const int64_t n3 = g.n3;
     const mfix_phi_fields &F = R.f;
     const auto *__restrict__ flag = to_global(F.flag);
@@ -82,38 +83,52 @@ __global__ void __launch_bounds__(kPhiBlock) k_conv_dif_phi(PhiArgs R) {
     const auto *__restrict__ fz_tb = to_global(F.fz_t_bar);
     int i, j, k;
     cell_ijk(g, c, i, j, k);
-    double ae = 0.0, aw = 0.0, an = 0.0, as = 0.0, at = 0.0, ab = 0.0;
-    if (fluid_at(flag[c])) {
-        const Nb n = neighbours(g, c);   // IM_OF.. of a fluid cell (set_index1a.f:85-222)
-        const int iw = nb_minus(i, g.ni, g.cyc_x), js = nb_minus(j, g.nj, g.cyc_y);
-        const double dc = dif[c];
-        // own east/north faces; EAST_OF is IP_OF unless that is a wall
-        const int64_t e = wall_at(flag[n.ip]) ? c : n.ip;
-        const int64_t no = wall_at(flag[n.jp]) ? c : n.jp;
-        const double dfe = avg_h(dc, dif[e], fx_e[i], fx_eb[i]) * (odx_e[i] * ayz[c]);
-        const double dfn = avg_h(dc, dif[no], fy_n[j], fy_nb[j]) * (ody_n[j] * axz[c]);
-        ae = u[c] >= 0.0 ? dfe : dfe - fe[c];
-        an = v[c] >= 0.0 ? dfn : dfn - fn[c];
-        // west/south: a fluid IM_OF writes this face as its own east face
-        // (EAST_OF(IM_OF) is this cell); otherwise this cell writes it with
-        // WEST_OF, which is the cell itself when IM_OF is a wall.  Both are
-        // the same expression.
-        const int64_t ws = wall_at(flag[n.im]) ? c : n.im;
-        const int64_t ss = wall_at(flag[n.jm]) ? c : n.jm;
-        const double dfw = avg_h(dif[ws], dc, fx_e[iw], fx_eb[iw]) * (odx_e[iw] * ayz[n.im]);
-        const double dfs = avg_h(dif[ss], dc, fy_n[js], fy_nb[js]) * (ody_n[js] * axz[n.jm]);
-        aw = u[n.im] >= 0.0 ? dfw + fe[n.im] : dfw;
-        as = v[n.jm] >= 0.0 ? dfs + fn[n.jm] : dfs;
-        if (g.do_k) {
-            const int kb = nb_minus(k, g.nk, g.cyc_z);
-            const int64_t t = wall_at(flag[n.kp]) ? c : n.kp;
-            const int64_t bs = wall_at(flag[n.km]) ? c : n.km;
-            const double dft = avg_h(dc, dif[t], fz_t[k], fz_tb[k]) * (ox[i] * odz_t[k] * axy[c]);
-            const double dfb = avg_h(dif[bs], dc, fz_t[kb], fz_tb[kb]) * (ox[i] * odz_t[kb] * axy[n.km]);
-            at = w[c] >= 0.0 ? dft : dft - ft[c];
-            ab = w[n.km] >= 0.0 ? dfb + ft[n.km] : dfb;
-        }
-    } else {
+    // IM_OF.. as a fluid cell has them (set_index1a.f:85-222).  They stay
+    // inside the grid for any cell, so a fluid row's coefficients are formed
+    // for every cell from loads issued together, without a branch the
+    // compiler could sink them into, and kept only on fluid rows (in 2-D
+    // KM = KP = the cell and the K faces are dropped).
+    const Nb n = neighbours(g, c);
+    const int iw = nb_minus(i, g.ni, g.cyc_x), js = nb_minus(j, g.nj, g.cyc_y), kb = nb_minus(k, g.nk, g.cyc_z);
+    const int fc = flag[c];
+    const int f_ip = flag[n.ip], f_jp = flag[n.jp], f_kp = flag[n.kp];
+    const int f_im = flag[n.im], f_jm = flag[n.jm], f_km = flag[n.km];
+    const double dc = dif[c];
+    const double d_ip = dif[n.ip], d_jp = dif[n.jp], d_kp = dif[n.kp];
+    const double d_im = dif[n.im], d_jm = dif[n.jm], d_km = dif[n.km];
+    const double uc = u[c], vc = v[c], wc = w[c], u_im = u[n.im], v_jm = v[n.jm], w_km = w[n.km];
+    const double fec = fe[c], fnc = fn[c], ftc = ft[c], fe_im = fe[n.im], fn_jm = fn[n.jm], ft_km = ft[n.km];
+    const double ayc = ayz[c], axzc = axz[c], axyc = axy[c];
+    const double ay_im = ayz[n.im], axz_jm = axz[n.jm], axy_km = axy[n.km];
+    const double gx = fx_e[i], gxb = fx_eb[i], ox_e = odx_e[i], gxw = fx_e[iw], gxwb = fx_eb[iw], ox_w = odx_e[iw];
+    const double gy = fy_n[j], gyb = fy_nb[j], oy_n = ody_n[j], gys = fy_n[js], gysb = fy_nb[js], oy_s = ody_n[js];
+    const double gz = fz_t[k], gzb = fz_tb[k], oz_t = odz_t[k], gzb_ = fz_t[kb], gzbb = fz_tb[kb], oz_b = odz_t[kb];
+    const double oxi = ox[i];
+    // own east/north/top faces; EAST_OF is IP_OF unless that is a wall
+    const double de = wall_at(f_ip) ? dc : d_ip;
+    const double dn = wall_at(f_jp) ? dc : d_jp;
+    const double dt = wall_at(f_kp) ? dc : d_kp;
+    const double dfe = avg_h(dc, de, gx, gxb) * (ox_e * ayc);
+    const double dfn = avg_h(dc, dn, gy, gyb) * (oy_n * axzc);
+    const double dft = avg_h(dc, dt, gz, gzb) * (oxi * oz_t * axyc);
+    // west/south/bottom: a fluid IM_OF writes this face as its own east face
+    // (EAST_OF(IM_OF) is this cell); otherwise this cell writes it with
+    // WEST_OF, which is the cell itself when IM_OF is a wall.  Both are the
+    // same expression.
+    const double dw = wall_at(f_im) ? dc : d_im;
+    const double ds = wall_at(f_jm) ? dc : d_jm;
+    const double db = wall_at(f_km) ? dc : d_km;
+    const double dfw = avg_h(dw, dc, gxw, gxwb) * (ox_w * ay_im);
+    const double dfs = avg_h(ds, dc, gys, gysb) * (oy_s * axz_jm);
+    const double dfb = avg_h(db, dc, gzb_, gzbb) * (oxi * oz_b * axy_km);
+    const bool fluid = fluid_at(fc), fk = fluid && g.do_k;
+    double ae = fluid ? (uc >= 0.0 ? dfe : dfe - fec) : 0.0;
+    double an = fluid ? (vc >= 0.0 ? dfn : dfn - fnc) : 0.0;
+    double at = fk ? (wc >= 0.0 ? dft : dft - ftc) : 0.0;
+    double aw = fluid ? (u_im >= 0.0 ? dfw + fe_im : dfw) : 0.0;
+    double as = fluid ? (v_jm >= 0.0 ? dfs + fn_jm : dfs) : 0.0;
+    double ab = fk ? (w_km >= 0.0 ? dfb + ft_km : dfb) : 0.0;
+    if (!fluid) {
         // a non-fluid row holds what a fluid cell whose IP_OF/JP_OF/KP_OF it
         // is scatters into it: that cell's east/north/top face, with
         // EAST_OF = the fluid cell itself when this row is a wall.  The
@@ -153,69 +168,70 @@ __global__ void __launch_bounds__(kPhiBlock) k_conv_dif_phi(PhiArgs R) {
     to_global(R.b)[c] = 0.0;
 }
 
-// SOURCE_PHI on the fluid rows.  Its dilute branch asks IS_AT_E(IMJK) etc.;
-// without internal surfaces a face is flagged only when a wall cell lies on
-// it (model/set_flags.f:416-470), and then WEST_OF (EAST_OF ..) is the cell
-// itself, whose EP is already below DIL_EP_s, so the test cannot change the
-// result and is left out.
+// SOURCE_PHI on the fluid rows.  Its dilute branch asks IS_AT_E(IMJK) etc.
+// Without internal surfaces SET_FLAGS (model/set_flags.f:403-470) leaves a
+// face flagged below 2000 only where a wall cell lies on it (a cyclic
+// boundary face is set back to 2000 when the ghost layer carries FLAG
+// 106/107, as SET_FLAGS gives it); a wall neighbour makes WEST_OF
+// (EAST_OF ..) the cell itself, whose EP is at most DIL_EP_s, so the test
+// cannot change the result and is left out.
 __global__ void __launch_bounds__(kPhiBlock) k_source_phi(SourceArgs R) {
     const DGrid &g = R.g;
     const int64_t c = (int64_t)blockIdx.x * kPhiBlock + threadIdx.x;
     if (c >= g.n3) return;
     const int64_t n3 = g.n3;
     const auto *__restrict__ flag = to_global(R.flag);
-    if (!fluid_at(flag[c])) return;
     const auto *__restrict__ ep = to_global(R.ep);
     const auto *__restrict__ phi = to_global(R.phi);
     auto *__restrict__ a = to_global(R.a);
     auto *__restrict__ b = to_global(R.b);
     const double dil = R.dil_ep_s;
-    if (ep[c] <= dil) {
-        const Nb n = neighbours(g, c);
-        auto side = [&](int64_t mp) { return ep[wall_at(flag[mp]) ? c : mp] > dil ? 1.0 : 0.0; };
-        const double aw = side(n.im), ae = side(n.ip), as = side(n.jm), an = side(n.jp);
-        const double ab = g.do_k ? side(n.km) : 0.0, at = g.do_k ? side(n.kp) : 0.0;
-        double a0 = -1.0, bc = 0.0;
-        if ((aw + ae + as + an + ab + at) == 0.0)
-            bc = -phi[c];
-        else
-            a0 = -(ae + aw + an + as + at + ab);
-        a[0 * n3 + c] = ab;
-        a[1 * n3 + c] = as;
-        a[2 * n3 + c] = aw;
-        a[3 * n3 + c] = a0;
-        a[4 * n3 + c] = ae;
-        a[5 * n3 + c] = an;
-        a[6 * n3 + c] = at;
-        b[c] = bc;
-        return;
-    }
+    // every load of a dense fluid row in one round; the row is formed for
+    // every cell and stored only where SOURCE_PHI writes it
+    const int fc = flag[c];
+    const double epc = ep[c], ph = phi[c], sp = to_global(R.s_p)[c], sc = to_global(R.s_c)[c], b0 = b[c];
     const double ab = a[0 * n3 + c], as = a[1 * n3 + c], aw = a[2 * n3 + c];
     const double ae = a[4 * n3 + c], an = a[5 * n3 + c], at = a[6 * n3 + c];
-    const double sp = to_global(R.s_p)[c], sc = to_global(R.s_c)[c];
-    const double ph = phi[c];
-    double bc = b[c];
-    double a0 = -(ae + aw + an + as + at + ab + sp);
-    if (bc < sc || ph == 0.0) {
-        bc = -sc + bc;
-    } else {
-        a0 = a0 - bc / ph;
-        bc = -sc;
+    const double a0 = -(ae + aw + an + as + at + ab + sp);
+    const bool keep_b = b0 < sc || ph == 0.0;
+    const double a0d = keep_b ? a0 : a0 - b0 / ph;
+    const double bd = keep_b ? -sc + b0 : -sc;
+    if (!fluid_at(fc)) return;
+    if (epc > dil) {
+        a[3 * n3 + c] = a0d;
+        b[c] = bd;
+        return;
     }
-    a[3 * n3 + c] = a0;
-    b[c] = bc;
+    const Nb n = neighbours(g, c);
+    auto side = [&](int64_t mp) { return ep[wall_at(flag[mp]) ? c : mp] > dil ? 1.0 : 0.0; };
+    const double sw = side(n.im), se = side(n.ip), ss = side(n.jm), sn = side(n.jp);
+    const double sb = g.do_k ? side(n.km) : 0.0, st = g.do_k ? side(n.kp) : 0.0;
+    double d0 = -1.0, db = 0.0;
+    if ((sw + se + ss + sn + sb + st) == 0.0)
+        db = -ph;
+    else
+        d0 = -(se + sw + sn + ss + st + sb);
+    a[0 * n3 + c] = sb;
+    a[1 * n3 + c] = ss;
+    a[2 * n3 + c] = sw;
+    a[3 * n3 + c] = d0;
+    a[4 * n3 + c] = se;
+    a[5 * n3 + c] = sn;
+    a[6 * n3 + c] = st;
+    b[c] = db;
 }
 
 __global__ void __launch_bounds__(kPhiBlock) k_under_relax_s(RelaxArgs R) {
     const int64_t c = (int64_t)blockIdx.x * kPhiBlock + threadIdx.x;
     if (c >= R.g.n3) return;
-    if (!fluid_at(to_global(R.flag)[c])) return;
     auto *__restrict__ a0 = to_global(R.a) + 3 * R.g.n3;
     auto *__restrict__ b = to_global(R.b);
-    const double ap = a0[c];
-    if (ap != -1.0) {
-        a0[c] = ap * R.f1;
-        b[c] = b[c] + ap * to_global(R.var)[c] * R.f2;
+    const int fc = to_global(R.flag)[c];
+    const double ap = a0[c], bc = b[c], var = to_global(R.var)[c];
+    const double na = ap * R.f1, nb = bc + ap * var * R.f2;
+    if (fluid_at(fc) && ap != -1.0) {
+        a0[c] = na;
+        b[c] = nb;
     }
 }
 
@@ -254,7 +270,8 @@ int mfix_conv_dif_phi_dev(const mfix_phi_fields *f, double *d_a_m, double *d_b_m
     R.f = *f;
     R.a = d_a_m;
     R.b = d_b_m;
-    hipLaunchKernelGGL(k_conv_dif_phi, dim3(phi_blocks(R.g)), dim3(kPhiBlock), 0, s, R);
+    const int64_t nblk = march_blocks(R.g.c2, R.g.nk, kPhiBlock);
+    hipLaunchKernelGGL(k_conv_dif_phi, dim3((unsigned)xcd_grid(nblk)), dim3(kPhiBlock), 0, s, R, nblk);
     return launched("CONV_DIF_PHI");
 }
 
