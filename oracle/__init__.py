@@ -264,7 +264,8 @@ def under_relax_s(g, a, b, var, flag, ur):
 
 # ---- driver of the compiled reference (ref_harness.f90) --------------------
 OPS = {"bicgs": 1, "matvec": 2, "msolve": 3, "msolve1": 4, "dot": 5, "dot2": 6,
-       "resid_c": 7, "resid_s": 8, "resid_pp": 9, "assemble_pp": 10, "phi": 11}
+       "resid_c": 7, "resid_s": 8, "resid_pp": 9, "assemble_pp": 10, "phi": 11,
+       "extract": 12, "write_ab_m": 13}
 
 
 def run_reference(op, g: OGrid, a, b, x, pc="DIAG", cmethod="RSRS", tol=1e-10, itmax=500, seed=12345,
@@ -275,7 +276,9 @@ def run_reference(op, g: OGrid, a, b, x, pc="DIAG", cmethod="RSRS", tol=1e-10, i
     pressure-correction assembly op pp = {flag, odt, ijk_p_g, fields}, the
     scalar transport op phi = {flag, stage, ur, fields} (fields: PHI_FIELDS,
     PHI_SOURCES and the 1-D geometry arrays; stage 1 CONV_DIF_PHI, 2
-    SOURCE_PHI, 3 UNDER_RELAX_S on the a, b given)."""
+    SOURCE_PHI, 3 UNDER_RELAX_S on the a, b given).  The dump writers
+    (extract: matrixExtract, write_ab_m: WRITE_AB_M) return the text files
+    they wrote under "files"."""
     if not REF_BIN.exists():
         raise FileNotFoundError(REF_BIN)
     n3 = n3_of(g)
@@ -305,15 +308,20 @@ def run_reference(op, g: OGrid, a, b, x, pc="DIAG", cmethod="RSRS", tol=1e-10, i
             f.write(np.ascontiguousarray(phi["flag"], dtype=np.int32).tobytes())
             for keys in (PHI_FIELDS + PHI_SOURCES, PHI_GEO_I, PHI_GEO_J, PHI_GEO_K):
                 f.write(np.stack([_f64(fd[k]) for k in keys]).tobytes())
-        elif OPS[op] >= 7:
+        elif 7 <= OPS[op] <= 9:
             f.write(np.array([int(resid["debug"]), int(resid["m"])], dtype=np.int32).tobytes())
             f.write(np.array([resid.get("tol", 0.0), resid.get("norm", 1.0)], dtype=np.float64).tobytes())
             f.write(np.ascontiguousarray(resid["flag"], dtype=np.int32).tobytes())
             eps = resid.get("ep_s")
             f.write(np.ascontiguousarray(np.zeros(n3) if eps is None else eps, dtype=np.float64).tobytes())
+    import tempfile
+    files = {}
     try:
-        subprocess.run([str(REF_BIN), str(fin), str(fout)], check=True, capture_output=True,
-                       env={**os.environ, "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS", "1")})
+        with tempfile.TemporaryDirectory(dir=workdir) as cwd:
+            subprocess.run([str(REF_BIN), str(fin), str(fout)], check=True, capture_output=True, cwd=cwd,
+                           env={**os.environ, "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS", "1")})
+            for q in Path(cwd).iterdir():
+                files[q.name] = q.read_text()
         raw = fout.read_bytes()
     finally:
         for p in (fin, fout):
@@ -329,4 +337,4 @@ def run_reference(op, g: OGrid, a, b, x, pc="DIAG", cmethod="RSRS", tol=1e-10, i
     b_out = np.frombuffer(raw[off:off + 8 * n3], dtype=np.float64).copy(); off += 8 * n3
     noise = np.frombuffer(raw[off:off + 8 * n3], dtype=np.float64).copy()
     return {"ier": int(ier), "iter_tot": int(iter_tot), "dot": dres, "x": y, "a": a_out, "b": b_out,
-            "noise": noise}
+            "noise": noise, "files": files}

@@ -42,6 +42,10 @@
 ! (model/source_phi.f:25) on the A/B read in; stage 3: UNDER_RELAX_S
 ! (model/under_relax.f:20) with VAR = PHI on the A/B read in.  FLAG_E/N/T
 ! follow SET_FLAGS (model/set_flags.f:400-470) without internal surfaces.
+! Op 12: INITEXTRACT(whole domain) + MATRIXEXTRACT(A_M, B_M, 0, 'HARNESS')
+! (model/dbg_mod.f:82, :228) write Am_HARNESS.csv and Bm_HARNESS.csv in the
+! working directory; op 13: WRITE_AB_M(A_M, B_M, 0) (model/write_ab_m.f:20)
+! writes to UNIT_LOG, opened as harness_ab_m.log in the working directory.
 program ref_harness
   use param, only: dimension_3
   use compar
@@ -66,6 +70,8 @@ program ref_harness
   use run, only: def_cor
   use is, only: is_defined
   use functions, only: wall_at, fluid_at, im_of, jm_of, km_of, ip_of, jp_of, kp_of
+  use dbg, only: initExtract, matrixExtract
+  use funits, only: unit_log
   implicit none
   integer :: shdr(1)
   double precision :: sur
@@ -230,7 +236,7 @@ program ref_harness
   end if
   close(10)
 
-  if (op >= 7) then
+  if (op >= 7 .and. op <= 11) then
      if (op == 10 .or. op == 11) then
         rhdr = 0
         rtol = 0d0; rnorm = 1d0
@@ -423,6 +429,26 @@ program ref_harness
      end if
      amat = a3(:, :, 0)
      bvec = b2(:, 0)
+     yvec = 0d0
+  case (12, 13)
+     dimension_m = 1
+     ijkmax3 = n3
+     allocate(a3(n3, -3:3, 0:1), b2(n3, 0:1))
+     a3 = 0d0; b2 = 0d0
+     a3(:, :, 0) = amat
+     b2(:, 0) = bvec
+     if (op == 12) then
+        ! the default region, given explicitly: initExtract's MERGE of an
+        ! absent optional argument (dbg_mod.f:108-113) dereferences it
+        call initExtract(imin3, imax3, jmin3, jmax3, kmin3, kmax3)
+        call matrixExtract(a3, b2, 0, 'HARNESS')
+     else
+        dmp_log = .true.
+        open(unit_log, file='harness_ab_m.log', status='replace')
+        call write_ab_m(a3, b2, 0)
+        close(unit_log)
+        dmp_log = .false.
+     end if
      yvec = 0d0
   case (7, 8, 9)
      rres = 0d0; rmax = 0d0; ijkr = 0
