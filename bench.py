@@ -414,6 +414,70 @@ def bench_phi(args):
     return 0
 
 
+def bench_sweep(args):
+    """One SOLVE_VEL_STAR stage of a SIMPLE iteration (model/solve_vel_star.f):
+    U, V and W of the gas and two solids phases, nine upwind momentum systems
+    on one n^3 grid that do not depend on each other's solutions, with MFIX's
+    momentum defaults (LINE, RSRS, TOL 1e-4, LEQ_IT 5, FP64).  The stage is
+    timed as nine sequential device solves (SOLVE_LIN_EQ call by call) and as
+    nine deferred calls solved by one sweep flush."""
+    from mfix_amd import leq, systems, _lib
+    n = args.n
+    leq.grid_init(n, n, n)
+    L = _lib.load()
+    names = [(f"{c}_{p}", vno) for p in ("g", "s1", "s2") for c, vno in (("U", 3), ("V", 4), ("W", 5))]
+    src = [systems.gen_system(systems.MOMENTUM, n, n, n, 6000 + q, nspheres=6) for q in range(len(names))]
+    n3 = src[0][1].size
+    dev = [(leq.DeviceArray(7 * n3), leq.DeviceArray(n3), leq.DeviceArray(n3)) for _ in names]
+    kw = dict(pc="LINE", cmethod="RSRS", tol=1e-4, itmax=5)
+    zeros = np.zeros(n3)
+
+    def load():
+        for (a, b), (da, db, dx) in zip(src, dev):
+            da.upload(a)
+            db.upload(b)
+            dx.upload(zeros)
+        _lib.check(L.mfix_leq_sync(), "sync")
+
+    def sequential():
+        return [leq.bicgs_dev(dx, da, db, **kw) for da, db, dx in dev]
+
+    def batched():
+        for (name, vno), (da, db, dx) in zip(names, dev):
+            leq.solve_lin_eq_deferred(name, vno, dx, da, db, kw["itmax"], kw["cmethod"], kw["tol"], kw["pc"])
+        return leq.sweep_flush()
+
+    times = {}
+    for label, fn in (("sequential", sequential), ("batched", batched)):
+        for _ in range(args.warmup):
+            load()
+            fn()
+        ts = []
+        for _ in range(args.steps):
+            load()
+            t = time.perf_counter()
+            out = fn()
+            ts.append(time.perf_counter() - t)
+        times[label] = (1e3 * sum(ts) / len(ts), out)
+    seq_ms, seq_out = times["sequential"]
+    bat_ms, bat_out = times["batched"]
+    iters = [d.stats["iterations"] for d in bat_out]
+    assert iters == [r.stats["iterations"] for r in seq_out]
+    line = {
+        "metric": "SOLVE_VEL_STAR stages/s (nine momentum equations of one SIMPLE iteration, batched)",
+        "value": 1e3 / bat_ms, "unit": "stages/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": bat_ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"U, V, W of the gas and two solids phases, upwind momentum {n}^3 (N3={n3}), "
+                               "LINE RSRS, TOL 1e-4, ITMAX 5 (MFIX momentum defaults)",
+                   "equations": len(names), "iterations": iters, "sequential_ms": seq_ms, "batched_ms": bat_ms,
+                   "speedup": seq_ms / bat_ms},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line))
+    return 0
+
+
 def bench_serve(args):
     """Poisson request streams through the continuous-batching server: sides
     lognormal (median 64, sigma 0.5) clipped to [24,160], 60% upwind momentum
@@ -492,7 +556,7 @@ def main(argv=None):
     ap.add_argument("--cpu-iters", type=int, default=6, help="reference iterations timed for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
-    ap.add_argument("--workload", choices=["mixed", "batch", "serve", "line", "resid", "pp", "phi"], default="mixed",
+    ap.add_argument("--workload", choices=["mixed", "batch", "serve", "line", "resid", "pp", "phi", "sweep"], default="mixed",
                     help="mixed: the 256^3 solve (headline); batch: 64 systems 16^3..128^3 in one batched "
                          "solve; serve: Poisson request streams through the continuous-batching server")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -530,6 +594,8 @@ def main(argv=None):
         return bench_pp(args)
     if args.workload == "phi":
         return bench_phi(args)
+    if args.workload == "sweep":
+        return bench_sweep(args)
     n = args.n
     t0 = time.time()
     a, b = systems.gen_system(systems.VOID_PRESSURE, n, n, n, 2000)

@@ -174,6 +174,38 @@ typedef struct mfix_leq_batch_item {
 int mfix_leq_batch_solve(int nsys, const mfix_leq_batch_item *items,
                          mfix_leq_stats *stats, void *stream);
 
+/* ---- SIMPLE-sweep batching ----------------------------------------------- */
+/* SOLVE_LIN_EQ(VNAME,VNO,VAR,A_M,B_M,M,ITMAX,METHOD,SWEEP,TOL1,PC,IER)
+ * (model/solve_lin_eq.f:17-195), deferred: the call is checked now (METHOD
+ * other than 2 -> MFIX_LEQ_EMETHOD, a PC other than LINE/DIAG/NONE ->
+ * MFIX_LEQ_EBADPC, as the reference stops there) and queued; it is solved at
+ * the next mfix_leq_sweep_flush together with every other queued call, as one
+ * batched solve on the current grid (one launch per phase for all systems).
+ * This is for the SOLVE_LIN_EQ calls of one DO_ITERATION stage whose systems
+ * do not depend on each other's solutions: SOLVE_VEL_STAR solves U, V and W
+ * of every phase into temporaries from the same old fields
+ * (model/solve_vel_star.f:86-99, :315-329), the species of SOLVE_SPECIES_EQ
+ * and the scalars of SOLVE_SCALAR_EQ likewise.  d_var, d_a_m (A_M(:,:,M))
+ * and d_b_m (B_M(:,M)) are device arrays that must stay untouched until the
+ * flush, one set per queued call (the reference reuses one A_M work array
+ * between equations; a batched stage keeps one per equation).  mixed = 1
+ * solves that system in mixed precision.  *ier is written and ITER_TOT(VNO)
+ * updated at the flush, as the call itself would have done.  At most 1024
+ * calls are queued (MFIX_LEQ_EBUSY beyond).  No counterpart in the
+ * reference (batching). */
+int mfix_solve_lin_eq_deferred(const char *vname, int vname_len, int vno,
+                               double *d_var, double *d_a_m, double *d_b_m,
+                               int itmax, int method, const char *sweep,
+                               double tol1, const char *pc, int mixed,
+                               int *ier);
+/* Solve every queued call in one batched solve and empty the queue (also
+ * when it fails).  stats: NULL or room for mfix_leq_sweep_pending() entries,
+ * in call order.  Returns 0, MFIX_LEQ_EDGTSV if a JK/IJ line sweep's DGTSV
+ * failed in one of them (IERs are still written), or another error. */
+int mfix_leq_sweep_flush(mfix_leq_stats *stats, void *stream);
+int mfix_leq_sweep_pending(void);   /* calls queued since the last flush */
+void mfix_leq_sweep_discard(void);  /* drop the queued calls unsolved */
+
 /* Autotuner: pick the block shape of the fused kernels for one grid size;
  * results are cached per (n3 class, precision).  Returns the chosen
  * threads-per-block and lines-per-block. */

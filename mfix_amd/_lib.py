@@ -136,6 +136,12 @@ _SIGS = {
     "mfix_solve_pp_g_dev": (C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                                       C.c_int, C.c_char_p, C.c_double, C.c_char_p, C.c_int, C.c_void_p, C.c_void_p,
                                       C.c_void_p, C.c_void_p]),
+    "mfix_solve_lin_eq_deferred": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                             C.c_int, C.c_int, C.c_char_p, C.c_double, C.c_char_p, C.c_int,
+                                             C.c_void_p]),
+    "mfix_leq_sweep_flush": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "mfix_leq_sweep_pending": (C.c_int, []),
+    "mfix_leq_sweep_discard": (None, []),
     "mfix_conv_dif_phi_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "mfix_source_phi_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
                                       C.c_void_p, C.c_void_p, C.c_void_p]),

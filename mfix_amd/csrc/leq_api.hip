@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <vector>
 
 #include "../../include/mfix_leq.h"
 #include "leq_context.h"
@@ -40,6 +41,17 @@ int sync_status(hipStream_t s, const char *what) {
     hipError_t e = hipStreamSynchronize(s);
     if (e == hipSuccess) e = hipGetLastError();
     return e == hipSuccess ? 0 : hip_fail(e, what);
+}
+
+// SOLVE_LIN_EQ calls deferred to the next sweep flush, in call order.
+struct Deferred {
+    mfix_leq_batch_item item;
+    int vno;
+    int *ier;
+};
+std::vector<Deferred> &sweep_queue() {
+    static std::vector<Deferred> q;
+    return q;
 }
 
 int copy_sync(void *dst, const void *src, size_t bytes, hipMemcpyKind kind) {
@@ -284,6 +296,66 @@ int mfix_leq_batch_solve(int nsys, const mfix_leq_batch_item *items, mfix_leq_st
     if ((rc = ctx().ensure_init())) return rc;
     if (nsys == 0) return 0;
     return solve_batch(nsys, items, stats, stream ? (hipStream_t)stream : ctx().stream);
+}
+
+int mfix_solve_lin_eq_deferred(const char *vname, int vname_len, int vno, double *d_var, double *d_a_m,
+                               double *d_b_m, int itmax, int method, const char *sweep, double tol1, const char *pc,
+                               int mixed, int *ier) {
+    (void)vname;
+    (void)vname_len;
+    // the checks SOLVE_LIN_EQ / LEQ_BICGS make at the call (solve_lin_eq.f:
+    // 139-190, leq_bicgs.f:83-87) happen here, before anything is queued
+    if (method != 2) return MFIX_LEQ_EMETHOD;
+    if (parse_pc(pc) < 0) return MFIX_LEQ_EBADPC;
+    if (!d_var || !d_a_m || !d_b_m || !ier || !sweep || vno < 1 || vno > 10 || itmax < 0) return MFIX_LEQ_EINVAL;
+    Context &C = ctx();
+    if (!C.have_grid) return MFIX_LEQ_ENOGRID;
+    std::vector<Deferred> &q = sweep_queue();
+    if ((int)q.size() >= kHostStates) return MFIX_LEQ_EBUSY;
+    Deferred d = {};
+    d.item.imax = C.grid.imax;
+    d.item.jmax = C.grid.jmax;
+    d.item.kmax = C.grid.no_k ? 1 : C.grid.kmax;
+    std::memcpy(d.item.pc, pc, 4);
+    for (int i = 0; i < 4; ++i) d.item.cmethod[i] = sweep[i] ? sweep[i] : ' ';
+    d.item.tol = tol1;
+    d.item.itmax = itmax;
+    d.item.mixed = mixed != 0;
+    d.item.d_var = d_var;
+    d.item.d_a_m = d_a_m;
+    d.item.d_b_m = d_b_m;
+    d.item.noise_seed = 0;   // the internal generator's default, as a single solve without options
+    d.vno = vno;
+    d.ier = ier;
+    q.push_back(d);
+    return 0;
+}
+
+int mfix_leq_sweep_pending(void) { return (int)sweep_queue().size(); }
+
+void mfix_leq_sweep_discard(void) { sweep_queue().clear(); }
+
+int mfix_leq_sweep_flush(mfix_leq_stats *stats, void *stream) {
+    std::vector<Deferred> q;
+    q.swap(sweep_queue());   // the queue is empty whatever the outcome
+    if (q.empty()) return 0;
+    int rc;
+    Context &C = ctx();
+    if ((rc = C.ensure_init())) return rc;
+    if (!C.have_grid) return MFIX_LEQ_ENOGRID;
+    const int n = (int)q.size();
+    std::vector<mfix_leq_batch_item> items((size_t)n);
+    std::vector<mfix_leq_grid> grids((size_t)n, C.grid);
+    std::vector<mfix_leq_stats> st((size_t)n);
+    for (int i = 0; i < n; ++i) items[(size_t)i] = q[(size_t)i].item;
+    rc = solve_batch(n, items.data(), st.data(), stream ? (hipStream_t)stream : C.stream, grids.data());
+    if (rc && rc != MFIX_LEQ_EDGTSV) return rc;
+    for (int i = 0; i < n; ++i) {
+        *q[(size_t)i].ier = st[(size_t)i].ier;
+        C.iter_tot[q[(size_t)i].vno - 1] += st[(size_t)i].iter_tot_increment;
+        if (stats) stats[i] = st[(size_t)i];
+    }
+    return rc;
 }
 
 int mfix_leq_autotune(int imax, int jmax, int kmax, int mixed, int *block, int *lines_per_block) {

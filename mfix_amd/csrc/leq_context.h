@@ -165,7 +165,9 @@ int solve_fp64(const DGrid &g, int pc, const char *cmethod, double tol, int itma
                const mfix_leq_opts *opts, mfix_leq_stats *stats, hipStream_t s, int minimize, int icheck);
 int solve_mixed(const DGrid &g, int pc, const char *cmethod, double tol, int itmax, double *x, double *a, double *b,
                 const mfix_leq_opts *opts, mfix_leq_stats *stats, hipStream_t s);
-int solve_batch(int nsys, const mfix_leq_batch_item *items, mfix_leq_stats *stats, hipStream_t s);
+// grids: one grid per item (NULL: a 3-D serial grid from each item's sizes)
+int solve_batch(int nsys, const mfix_leq_batch_item *items, mfix_leq_stats *stats, hipStream_t s,
+                const mfix_leq_grid *grids = nullptr);
 template <typename T>
 int launch_msolve_line(const DGrid &g, const Shape &sh, const char *cmethod, const T *a, const T *r, T *z,
                        T *scratch, const SolverState *st, int skip_on_sexit, int *fatal_flag, hipStream_t s,

@@ -846,7 +846,8 @@ int solve_mixed(const DGrid &g, int pc, const char *cmethod, double tol, int itm
 // advance together, one launch per phase covering all their systems, in
 // captured chunks.  Threads per block are shared by a table; cells per
 // thread follow each system's size.
-int solve_batch(int nsys, const mfix_leq_batch_item *items, mfix_leq_stats *stats, hipStream_t s) {
+int solve_batch(int nsys, const mfix_leq_batch_item *items, mfix_leq_stats *stats, hipStream_t s,
+                const mfix_leq_grid *grids) {
     Context &C = ctx();
     int rc;
     std::vector<SysDesc> descs((size_t)nsys);
@@ -857,15 +858,22 @@ int solve_batch(int nsys, const mfix_leq_batch_item *items, mfix_leq_stats *stat
     for (int q = 0; q < nsys; ++q) {
         const mfix_leq_batch_item &it = items[q];
         mfix_leq_grid hg = {};
-        hg.imax = it.imax;
-        hg.jmax = it.jmax;
-        hg.kmax = it.kmax;
-        hg.is_serial = 1;
-        hg.icheck_bicgs = 1;
+        if (grids) {
+            hg = grids[q];
+        } else {
+            hg.imax = it.imax;
+            hg.jmax = it.jmax;
+            hg.kmax = it.kmax;
+            hg.is_serial = 1;
+            hg.icheck_bicgs = 1;
+        }
         const DGrid g = make_dgrid(hg);
         const int pc = parse_pc(it.pc);
         char cm[5] = {it.cmethod[0], it.cmethod[1], it.cmethod[2], it.cmethod[3], 0};
-        descs[q] = make_desc(g, pc, cm, it.tol, it.itmax, it.d_var, it.d_a_m, it.d_b_m, nullptr, 0, 1, 0.1);
+        // the FP64 rows follow the grid's MINIMIZE_DOTPRODUCTS / ICHECK_BICGS
+        // as solve_fp64 does; the mixed rows use the fixed schedule of solve_mixed
+        descs[q] = make_desc(g, pc, cm, it.tol, it.itmax, it.d_var, it.d_a_m, it.d_b_m, nullptr,
+                             it.mixed ? 0 : hg.minimize_dotproducts, it.mixed ? 1 : std::max(1, hg.icheck_bicgs), 0.1);
         descs[q].seed = it.noise_seed;
         const Shape sh = C.shape_for(g.n3, it.mixed != 0);
         parts[q] = make_part(g.n3, bs, sh.bs == bs ? sh.cpt : 1);

@@ -416,3 +416,60 @@ def under_relax_s_dev(var: "DeviceArray", flag: "DeviceArray", ur: float, a: "De
     check(L.mfix_under_relax_s_dev(var.ptr, flag.ptr, ur, a.ptr, b.ptr, None), "mfix_under_relax_s_dev")
     if sync:
         check(L.mfix_leq_sync(), "mfix_leq_sync")
+
+
+# ---- SIMPLE-sweep batching (deferred SOLVE_LIN_EQ, model/solve_lin_eq.f) ----
+
+class Deferred:
+    """A queued SOLVE_LIN_EQ call; ier and stats are filled at the flush."""
+
+    def __init__(self, vname: str, vno: int, var: "DeviceArray", a_m: "DeviceArray", b_m: "DeviceArray"):
+        self.vname, self.vno = vname, vno
+        self.arrays = (var, a_m, b_m)   # kept alive until the flush
+        self._ier = C.c_int(0)
+        self.stats: dict | None = None
+
+    @property
+    def ier(self) -> int:
+        return self._ier.value
+
+
+_queued: list[Deferred] = []
+
+
+def solve_lin_eq_deferred(vname: str, vno: int, var: "DeviceArray", a_m: "DeviceArray", b_m: "DeviceArray",
+                          itmax: int, sweep: str = "RSRS", tol: float = 1e-4, pc: str = "LINE", *,
+                          method: int = 2, mixed: bool = False) -> Deferred:
+    """Queue SOLVE_LIN_EQ(VNAME, VNO, VAR, A_M(:,:,M), B_M(:,M), ...) on
+    resident arrays for the next sweep_flush (argument checks happen now)."""
+    d = Deferred(vname, vno, var, a_m, b_m)
+    rc = _lib_().mfix_solve_lin_eq_deferred(vname.encode(), len(vname), vno, var.ptr, a_m.ptr, b_m.ptr, itmax,
+                                            method, _pad4(sweep), tol, _pad4(pc), int(mixed), C.byref(d._ier))
+    if rc == -101:
+        raise SolverAbort("preconditioner option not found - check mfix.dat and readme")
+    check(rc, "mfix_solve_lin_eq_deferred")
+    _queued.append(d)
+    return d
+
+
+def sweep_pending() -> int:
+    return _lib_().mfix_leq_sweep_pending()
+
+
+def sweep_discard() -> None:
+    _lib_().mfix_leq_sweep_discard()
+    _queued.clear()
+
+
+def sweep_flush() -> list[Deferred]:
+    """Solve the queued calls together; returns them with ier and stats set."""
+    n = sweep_pending()
+    st = (Stats * max(n, 1))()
+    done = list(_queued)
+    _queued.clear()
+    rc = _lib_().mfix_leq_sweep_flush(st, None)
+    if rc in (0, -106):   # with a DGTSV failure every IER is still written
+        for q, d in enumerate(done[:n]):
+            d.stats = st[q].as_dict()
+    check(rc, "mfix_leq_sweep_flush")
+    return done

@@ -1,0 +1,73 @@
+"""SIMPLE-sweep batching on the GPU: SOLVE_LIN_EQ calls queued with
+mfix_solve_lin_eq_deferred and solved together at mfix_leq_sweep_flush give,
+call by call, the single device solve's result bit for bit (the batched
+systems run the same kernels with their own state), the same IER and the
+same ITER_TOT(VNO) accounting (model/leqsol_mod.f:17), on 3-D, 2-D and
+cyclic grids and with FP64 and mixed-precision calls in one flush."""
+import numpy as np
+import pytest
+
+from mfix_amd import leq, systems
+from mfix_amd.leq import DeviceArray
+
+pytestmark = pytest.mark.gpu
+
+# a SOLVE_VEL_STAR-like stage: U, V, W of the gas (VNO 3, 4, 5), one solids
+# phase's U (VNO 3) and a scalar (VNO 9) and a pressure-like system
+STAGE = [("U_g", 3, 1, "LINE", "RSRS", 5, 1e-4, False), ("V_g", 4, 1, "LINE", "RSRS", 5, 1e-4, False),
+         ("W_g", 5, 1, "LINE", "RSRS", 5, 1e-4, False), ("U_s", 3, 1, "LINE", "ISIS", 5, 1e-4, True),
+         ("Scalar 1", 9, 2, "DIAG", "RSRS", 50, 1e-8, False), ("P_like", 1, 0, "DIAG", "RSRS", 200, 1e-10, True)]
+
+
+def _grid_system(kind, im, jm, km, no_k, seed):
+    if no_k:
+        from tests.cases import make_system
+        return make_system(kind, im, jm, 1, no_k=True, seed=seed)
+    return systems.gen_system(kind, im, jm, km, seed, nspheres=6)
+
+
+@pytest.mark.parametrize("dims,no_k,cyc", [((20, 18, 16), False, (0, 0, 0)), ((40, 30, 1), True, (0, 0, 0)),
+                                           ((18, 16, 14), False, (1, 0, 1))])
+def test_flush_equals_sequential_solves(gpu, dims, no_k, cyc):
+    im, jm, km = dims
+    print(f"sweep flush vs single solves {dims} no_k={no_k} cyc={cyc}", flush=True)
+    gpu.grid_init(im, jm, km, no_k=no_k, cyclic=tuple(bool(c) for c in cyc))
+    sysdata = [_grid_system(kind, im, jm, km, no_k, 700 + q) for q, (_, _, kind, *_rest) in enumerate(STAGE)]
+    # sequential: one device solve per call
+    gpu.reset_iter_tot()
+    seq = []
+    for (vname, vno, kind, pc, sweep, itmax, tol, mixed), (a, b) in zip(STAGE, sysdata):
+        da, db, dx = DeviceArray.from_host(a), DeviceArray.from_host(b), DeviceArray.from_host(np.zeros(b.size))
+        r = gpu.bicgs_dev(dx, da, db, pc=pc, cmethod=sweep, tol=tol, itmax=itmax, mixed=mixed)
+        seq.append((dx.download(), r))
+    # the device entry has no VNO: ITER_TOT(VNO) is what LEQ_BICGS would add
+    tot_seq = {v: 0 for v in (1, 3, 4, 5, 9)}
+    for (vname, vno, *_rest), (_, r) in zip(STAGE, seq):
+        tot_seq[vno] += r.stats["iter_tot_increment"]
+    assert any(tot_seq.values())
+    # deferred: queued, then one flush
+    gpu.reset_iter_tot()
+    held, calls = [], []
+    for (vname, vno, kind, pc, sweep, itmax, tol, mixed), (a, b) in zip(STAGE, sysdata):
+        da, db, dx = DeviceArray.from_host(a), DeviceArray.from_host(b), DeviceArray.from_host(np.zeros(b.size))
+        held.append(dx)
+        calls.append(gpu.solve_lin_eq_deferred(vname, vno, dx, da, db, itmax, sweep, tol, pc, mixed=mixed))
+    assert gpu.sweep_pending() == len(STAGE)
+    done = gpu.sweep_flush()
+    assert gpu.sweep_pending() == 0
+    for (x_seq, r_seq), d, dx in zip(seq, done, held):
+        assert d.ier == r_seq.ier
+        assert d.stats["iterations"] == r_seq.stats["iterations"]
+        np.testing.assert_array_equal(dx.download(), x_seq)
+    assert {v: gpu.iter_tot(v) for v in (1, 3, 4, 5, 9)} == tot_seq
+
+
+def test_flush_of_nothing_and_reuse(gpu):
+    gpu.grid_init(12, 10, 8)
+    assert gpu.sweep_flush() == []
+    a, b = systems.gen_system(1, 12, 10, 8, 3, nspheres=6)
+    for rep in range(2):   # the queue empties at a flush and takes new calls
+        da, db, dx = DeviceArray.from_host(a), DeviceArray.from_host(b), DeviceArray.from_host(np.zeros(b.size))
+        gpu.solve_lin_eq_deferred("U_g", 3, dx, da, db, 20, "RSRS", 1e-8, "LINE")
+        (d,) = gpu.sweep_flush()
+        assert d.ier == 0 and d.stats["iterations"] > 0
