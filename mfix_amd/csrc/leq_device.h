@@ -398,6 +398,51 @@ __device__ __forceinline__ void block_partial(double (&v)[NV], PD partials, int6
     }
 }
 
+// block_partial without a barrier at the block's end: each wave leaves its
+// sums in LDS and counts itself in; the last wave of the block to arrive adds
+// the wave sums in wave order (the order block_sum uses, so the partials are
+// the same bits) and stores them.  Waves that finish early retire at once
+// instead of holding their slots until the slowest wave of the block reaches
+// a barrier.  *cnt must be zero before any wave arrives: partial_begin, at
+// the kernel's start.
+template <int BS>
+__device__ __forceinline__ void partial_begin(unsigned *cnt) {
+    if (BS > 64) {
+        if (threadIdx.x == 0) *cnt = 0u;
+        __syncthreads();
+    }
+}
+
+template <int BS, int NV, typename PD>
+__device__ __forceinline__ void block_partial_last(double (&v)[NV], PD partials, int64_t nblk, int64_t blk,
+                                                   double *lds, unsigned *cnt) {
+    constexpr int NW = BS / 64;
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        double s = v[q];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+        v[q] = s;
+    }
+    if ((threadIdx.x & 63) != 0) return;
+    if (NW == 1) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) partials[q * nblk + blk] = v[q];
+        return;
+    }
+    const int w = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NV; ++q) lds[q * NW + w] = v[q];
+    const unsigned before = __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (before != (unsigned)(NW - 1)) return;
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        double s = lds[q * NW + 0];
+        for (int u = 1; u < NW; ++u) s += lds[q * NW + u];
+        partials[q * nblk + blk] = s;
+    }
+}
+
 // Finish step, run by nfb <= kFinishBlocks blocks of kFinishBlock threads
 // (fb = this block's index among them).  Block fb sums the partials of
 // slice fb in a fixed order into level-2 sums; the

@@ -206,10 +206,12 @@ __global__ void __launch_bounds__(BS / Group16<T>::N) k_p_update(const Sys<T> *_
 template <int BS, typename T, int MODE>
 __global__ void __launch_bounds__(BS) k_matvec_dots(const Sys<T> *__restrict__ sys, const int2 *__restrict__ map) {
     __shared__ double lds[(BS / 64) * 2];
+    __shared__ unsigned arrived;
     SYS_BLOCK(T)
     auto *__restrict__ st = to_global(S.st);
     if (st->done) return;
     if (MODE == 1 && st->s_exit) return;
+    partial_begin<BS>(&arrived);
     double v[2] = {0.0, 0.0};
     const bool all = MODE == 1 ? (g.dot_all && !st->minimize) : g.dot_all;
     auto *__restrict__ a = to_global(S.a);
@@ -236,9 +238,9 @@ __global__ void __launch_bounds__(BS) k_matvec_dots(const Sys<T> *__restrict__ s
     else body(Operand<T, OP_PLAIN, gptr<const T>>{xin, nullptr});
     if (MODE == 0) {
         double v1[1] = {v[0]};
-        block_partial<BS, 1>(v1, to_global(S.partials), P.nblk, blk, lds);
+        block_partial_last<BS, 1>(v1, to_global(S.partials), P.nblk, blk, lds, &arrived);
     } else {
-        block_partial<BS, 2>(v, to_global(S.partials), P.nblk, blk, lds);
+        block_partial_last<BS, 2>(v, to_global(S.partials), P.nblk, blk, lds, &arrived);
     }
 }
 
