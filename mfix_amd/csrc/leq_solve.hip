@@ -656,8 +656,9 @@ static int capture_chunk(Group<double> *G64, Group<float> *G32, int iters, hipSt
 
 // Drive both tables (either may be null) until every system is done.  The
 // running count is polled after each chunk; chunks grow 4, 8, ... 64
-// iterations (direct launches) or are a fixed captured graph.  Launches
-// after a system is done return at once, so overshooting is harmless.
+// iterations (direct launches) or are a captured graph of 16, never more
+// than ITMAX.  Launches after a system is done return at once, so
+// overshooting is harmless, but each still dispatches its blocks.
 static int run_tables(Group<double> *G64, Group<float> *G32, int max_iters, hipStream_t s, bool graphs) {
     Context &C = ctx();
     int rc;
@@ -677,9 +678,12 @@ static int run_tables(Group<double> *G64, Group<float> *G32, int max_iters, hipS
     uint64_t *exec_version = G32 ? &G32->exec_version : &G64->exec_version;
     const void **exec_sys = G32 ? &G32->exec_sys : &G64->exec_sys;
     const void *sys_key = G32 ? (const void *)G32->d_sys : (const void *)G64->d_sys;
-    const int kGraphIters = 16;
+    // no chunk runs past ITMAX: with MFIX's LEQ_IT of 5-20 a fixed 16 would
+    // dispatch every block of up to 11 idle iterations per solve
+    const int cap = std::max(1, std::min(max_iters, 64));
+    const int kGraphIters = std::min(16, cap);
     int64_t issued = 0;
-    int chunk = 4;
+    int chunk = std::min(4, cap);
     for (;;) {
         if (G64) enqueue_count<double>(*G64, G64->iter, d_cnt + 0, d_flags64, s);
         if (G32) enqueue_count<float>(*G32, G32->iter, d_cnt + 1, d_flags32, s);
@@ -713,7 +717,7 @@ static int run_tables(Group<double> *G64, Group<float> *G32, int max_iters, hipS
                 if (G32) enqueue_iterations<float>(*G32, G32->iter, 1, s);
             }
             issued += chunk;
-            chunk = std::min(chunk * 2, 64);
+            chunk = std::min(chunk * 2, cap);
         }
     }
 }
