@@ -504,23 +504,27 @@ __global__ void __launch_bounds__(kFinishBlock) k_finish_sys(const Sys<T> *__res
 
 // Red-black line phase along J for every LINE system of a table (RSRS and
 // the 2-D R sweep).  Block b takes lines [map[b].y*64, +64) of colour
-// `param` of system map[b].x, numbered densely by color_line, with two
+// `param` of system map[b].x, numbered densely by color_line, with three
 // waves: lane l of wave 0 runs DGTSV on line l (the operations and order of
-// solve_line, so the result is bit-identical); wave 1 feeds it.  Lines along
-// J are contiguous, so wave 1 reads rows in tiles of 8 - element k of its
-// lane is row (lane & 7) of line (lane >> 3) + 8k, each load instruction
-// fetching 8 lines x 64 contiguous bytes - forms the right-hand side (B
-// minus the I/K neighbour terms, whose offsets are constant along a J-line)
-// and stages the tile in LDS while wave 0 eliminates the previous tile.
-// Finished rows go to scratch in [row][lane] order; the back solve reads them
-// in groups of 8 rows, the next group's loads in flight while the current
-// one is solved, and wave 1 writes the solution out through LDS, coalesced.
-// which = 0 preconditions P into Phat, 1 S into Shat.
+// solve_line, so the result is bit-identical); waves 1 and 2 feed it.  Lines
+// along J are contiguous, so a loader wave reads rows in groups of 8 -
+// element k of its lane is row (lane & 7) of line (lane >> 3) + 8k, each
+// load instruction fetching 8 lines x 64 contiguous bytes - forms the
+// right-hand side (B minus the I/K neighbour terms, whose offsets are
+// constant along a J-line) and stages it in LDS.  A tile is 16 rows, 8 from
+// each loader, staged while wave 0 eliminates the previous tile: one memory
+// round trip per 16 rows (a 130-row line takes 9, where 8-row tiles took 17
+// and left the phase waiting on them).  Finished rows go to scratch in
+// [row][lane] order; the back solve reads them in groups of 8 rows, the next
+// group's loads in flight while the current one is solved, and wave 1 writes
+// the solution out through LDS, coalesced.  which = 0 preconditions P into
+// Phat, 1 S into Shat.
 template <typename T>
-__global__ void __launch_bounds__(128) k_line_color_sys(const Sys<T> *__restrict__ sys, const int2 *__restrict__ map,
+__global__ void __launch_bounds__(192) k_line_color_sys(const Sys<T> *__restrict__ sys, const int2 *__restrict__ map,
                                                         int param, int which) {
-    constexpr int R = 8;
-    __shared__ T tl[2][4][R][64];  // dl, d, du, bb: R rows x 64 lines, double buffered
+    constexpr int R = 8;           // rows per loader wave and per back-solve group
+    constexpr int RT = 2 * R;      // rows per elimination tile
+    __shared__ T tl[2][4][RT][64]; // dl, d, du, bb: RT rows x 64 lines, double buffered
     __shared__ T xs[R][64];        // solution rows staged for the stores
     __shared__ int linfo[64];      // DGTSV INFO per line
     const int2 e = map[blockIdx.x];
@@ -532,7 +536,8 @@ __global__ void __launch_bounds__(128) k_line_color_sys(const Sys<T> *__restrict
     const LineGeom L = line_geom(g, AX_J);
     const int n = L.n;
     const int64_t n3 = g.n3;
-    const int wave = threadIdx.x >> 6;  // 0: elimination and back solve, 1: loads and stores
+    const int wave = threadIdx.x >> 6;  // 0: elimination and back solve; 1, 2: loads (1: stores)
+    const int ro = wave >= 1 ? R * (wave - 1) : 0;  // a loader's first row within a tile
     const int lane = threadIdx.x & 63;
     const int do_k = g.do_k;
     constexpr int pd = DiagPlane<T>::value;
@@ -591,36 +596,36 @@ __global__ void __launch_bounds__(128) k_line_color_sys(const Sys<T> *__restrict
         if (do_k) bb = tb[k] - t1m[k] * v1m[k] - t1p[k] * v1p[k] - t2m[k] * v2m[k] - t2p[k] * v2p[k];    \
         else bb = tb[k] - t1m[k] * v1m[k] - t1p[k] * v1p[k];                                             \
         const int j = (lane >> 3) + 8 * k;                                                               \
-        tl[BUF][0][rsub][j] = tdl[k];                                                                    \
-        tl[BUF][1][rsub][j] = td[k];                                                                     \
-        tl[BUF][2][rsub][j] = tdu[k];                                                                    \
-        tl[BUF][3][rsub][j] = bb;                                                                        \
+        tl[BUF][0][ro + rsub][j] = tdl[k];                                                               \
+        tl[BUF][1][ro + rsub][j] = td[k];                                                                \
+        tl[BUF][2][ro + rsub][j] = tdu[k];                                                               \
+        tl[BUF][3][ro + rsub][j] = bb;                                                                   \
     }
 #define MFIX_LINE_TILE(Q0, BUF) \
     MFIX_LINE_LOAD(Q0)          \
     MFIX_LINE_STAGE(BUF)
-    const int ntiles = (n + R - 1) / R;
-    if (wave == 1) {
-        MFIX_LINE_TILE(0, 0)
+    const int ntiles = (n + RT - 1) / RT;
+    if (wave >= 1) {
+        MFIX_LINE_TILE(ro, 0)
     }
     __syncthreads();
     T d0 = (T)0, du0 = (T)0, b0 = (T)0;
     int info = 0;
     for (int tt = 0; tt < ntiles; ++tt) {
         const int buf = tt & 1;
-        if (wave == 1) {
+        if (wave >= 1) {
             if (tt + 1 < ntiles) {
                 if (buf) {
-                    MFIX_LINE_TILE((tt + 1) * R, 0)
+                    MFIX_LINE_TILE((tt + 1) * RT + ro, 0)
                 } else {
-                    MFIX_LINE_TILE((tt + 1) * R, 1)
+                    MFIX_LINE_TILE((tt + 1) * RT + ro, 1)
                 }
             }
         } else {
             // DGTSV elimination of this lane's line over the tile
             // (DGTSV.f:107-163, the operations of solve_line in the same order)
-            const int q0 = tt * R;
-            for (int r = 0; r < R; ++r) {
+            const int q0 = tt * RT;
+            for (int r = 0; r < RT; ++r) {
                 const int q = q0 + r;
                 if (q >= n) break;
                 const T dl1 = tl[buf][0][r][lane];

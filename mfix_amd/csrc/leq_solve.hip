@@ -551,7 +551,7 @@ static void line_sweeps(const Group<T> &G, const MapSet &M, int which, hipStream
         if (M.lmap[q].empty()) continue;
         for (int param : {1, 0}) {
             pf.begin(s);
-            hipLaunchKernelGGL(k_line_color_sys<T>, dim3((unsigned)M.lmap[q].size()), dim3(128), 0, s, G.d_sys,
+            hipLaunchKernelGGL(k_line_color_sys<T>, dim3((unsigned)M.lmap[q].size()), dim3(192), 0, s, G.d_sys,
                                M.d_lmap[q], param, which);
             pf.end(s, 0.5 * M.cells * 13.0 * sizeof(T));
         }
