@@ -180,7 +180,8 @@ int mfix_leq_batch_solve(int nsys, const mfix_leq_batch_item *items,
  * other than 2 -> MFIX_LEQ_EMETHOD, a PC other than LINE/DIAG/NONE ->
  * MFIX_LEQ_EBADPC, as the reference stops there) and queued; it is solved at
  * the next mfix_leq_sweep_flush together with every other queued call, as one
- * batched solve on the current grid (one launch per phase for all systems).
+ * batched solve, each on the grid current at its call (one launch per phase
+ * for all systems).
  * This is for the SOLVE_LIN_EQ calls of one DO_ITERATION stage whose systems
  * do not depend on each other's solutions: SOLVE_VEL_STAR solves U, V and W
  * of every phase into temporaries from the same old fields

@@ -46,6 +46,7 @@ int sync_status(hipStream_t s, const char *what) {
 // SOLVE_LIN_EQ calls deferred to the next sweep flush, in call order.
 struct Deferred {
     mfix_leq_batch_item item;
+    mfix_leq_grid grid;   // the grid current at the call
     int vno;
     int *ier;
 };
@@ -325,6 +326,7 @@ int mfix_solve_lin_eq_deferred(const char *vname, int vname_len, int vno, double
     d.item.d_a_m = d_a_m;
     d.item.d_b_m = d_b_m;
     d.item.noise_seed = 0;   // the internal generator's default, as a single solve without options
+    d.grid = C.grid;
     d.vno = vno;
     d.ier = ier;
     q.push_back(d);
@@ -342,12 +344,14 @@ int mfix_leq_sweep_flush(mfix_leq_stats *stats, void *stream) {
     int rc;
     Context &C = ctx();
     if ((rc = C.ensure_init())) return rc;
-    if (!C.have_grid) return MFIX_LEQ_ENOGRID;
     const int n = (int)q.size();
     std::vector<mfix_leq_batch_item> items((size_t)n);
-    std::vector<mfix_leq_grid> grids((size_t)n, C.grid);
+    std::vector<mfix_leq_grid> grids((size_t)n);
     std::vector<mfix_leq_stats> st((size_t)n);
-    for (int i = 0; i < n; ++i) items[(size_t)i] = q[(size_t)i].item;
+    for (int i = 0; i < n; ++i) {
+        items[(size_t)i] = q[(size_t)i].item;
+        grids[(size_t)i] = q[(size_t)i].grid;
+    }
     rc = solve_batch(n, items.data(), st.data(), stream ? (hipStream_t)stream : C.stream, grids.data());
     if (rc && rc != MFIX_LEQ_EDGTSV) return rc;
     for (int i = 0; i < n; ++i) {

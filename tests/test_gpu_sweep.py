@@ -71,3 +71,22 @@ def test_flush_of_nothing_and_reuse(gpu):
         gpu.solve_lin_eq_deferred("U_g", 3, dx, da, db, 20, "RSRS", 1e-8, "LINE")
         (d,) = gpu.sweep_flush()
         assert d.ier == 0 and d.stats["iterations"] > 0
+
+
+def test_each_call_keeps_its_grid(gpu):
+    """A flush solves every queued call on the grid that was current at the
+    call, so calls on different grids can share one flush."""
+    grids = [(12, 10, 8), (9, 14, 11)]
+    singles, held = [], []
+    for q, (im, jm, km) in enumerate(grids):
+        a, b = systems.gen_system(1, im, jm, km, 40 + q, nspheres=6)
+        gpu.grid_init(im, jm, km)
+        da, db, dx = DeviceArray.from_host(a), DeviceArray.from_host(b), DeviceArray.from_host(np.zeros(b.size))
+        singles.append((dx, gpu.bicgs_dev(dx, da, db, pc="LINE", cmethod="RSRS", tol=1e-8, itmax=50)))
+        da, db, dx = DeviceArray.from_host(a), DeviceArray.from_host(b), DeviceArray.from_host(np.zeros(b.size))
+        held.append(dx)
+        gpu.solve_lin_eq_deferred("U_g", 3, dx, da, db, 50, "RSRS", 1e-8, "LINE")
+    done = gpu.sweep_flush()
+    for (x_single, r), d, dx in zip(singles, done, held):
+        assert d.ier == r.ier and d.stats["iterations"] == r.stats["iterations"]
+        np.testing.assert_array_equal(dx.download(), x_single.download())
