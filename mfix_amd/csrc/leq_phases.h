@@ -625,11 +625,21 @@ __global__ void __launch_bounds__(192) k_line_color_sys(const Sys<T> *__restrict
             // DGTSV elimination of this lane's line over the tile
             // (DGTSV.f:107-163, the operations of solve_line in the same order)
             const int q0 = tt * RT;
+            // row r+1's values are read while row r is eliminated, so the LDS
+            // latency stays off the serial chain of divisions
+            T ndl = tl[buf][0][0][lane], nd = tl[buf][1][0][lane], ndu = tl[buf][2][0][lane];
+            T nb = tl[buf][3][0][lane];
             for (int r = 0; r < RT; ++r) {
                 const int q = q0 + r;
                 if (q >= n) break;
-                const T dl1 = tl[buf][0][r][lane];
-                T d1 = tl[buf][1][r][lane], du1 = tl[buf][2][r][lane], b1 = tl[buf][3][r][lane];
+                const T dl1 = ndl;
+                T d1 = nd, du1 = ndu, b1 = nb;
+                if (r + 1 < RT) {
+                    ndl = tl[buf][0][r + 1][lane];
+                    nd = tl[buf][1][r + 1][lane];
+                    ndu = tl[buf][2][r + 1][lane];
+                    nb = tl[buf][3][r + 1][lane];
+                }
                 if (q == 0) {
                     d0 = d1;
                     du0 = du1;
