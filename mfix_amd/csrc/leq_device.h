@@ -191,12 +191,12 @@ template <> struct VecOf<double, 4> { using type = d4v; };
 template <typename T> struct Group16 { static constexpr int N = 16 / sizeof(T); };
 template <typename E, int N> using vec_t = typename VecOf<E, N>::type;
 
+// The update passes read each array once, so their loads carry the
+// non-temporal hint; their stores stay plain: the next pass reads what they
+// store, and at 256^3 (FP32) plain stores ran the solve 0.7 % faster than
+// non-temporal ones (574.8 against 578.8 ms, same iterations).
 template <typename E, int N, typename P>
 __device__ __forceinline__ vec_t<E, N> ldv(P p) {
-    return *(const __attribute__((address_space(1))) vec_t<E, N> *)p;
-}
-template <typename E, int N, typename P>
-__device__ __forceinline__ vec_t<E, N> ldv_nt(P p) {
     return __builtin_nontemporal_load((const __attribute__((address_space(1))) vec_t<E, N> *)p);
 }
 template <typename E, int N, typename P>

@@ -222,7 +222,7 @@ __global__ void __launch_bounds__(BS) k_matvec_dots(const Sys<T> *__restrict__ s
     auto body = [&](auto X) {
         auto cell = [&](int64_t c) {
             const Nb n = neighbours_fast(g, c);
-            const T wc = w[c];
+            const T wc = ld_stream(&w[c]);
             const T yy = Stencil<T>::row(g, a, X, c, n);
             y[c] = yy;
             if (all || n.interior) {
