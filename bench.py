@@ -651,8 +651,8 @@ def main(argv=None):
     value = args.steps / total
     fp64_value = len(fp64_times) / sum(fp64_times) if fp64_times else None
 
-    # dominant kernel: the first FP32 stencil pass k_matvec_dots<float, 0>
-    # (V = A*(P/A0) and <Rtilde,V>): 6 off-diagonal planes + row excess
+    # dominant kernel: the first FP32 stencil pass k_matvec_dots2<float, 0>
+    # (V = A*(P/A0) and <Rtilde,V>, two cells per thread): 6 off-diagonal planes + row excess
     # + operand + dot partner + result = 10*4 = 40 B/cell (HIP events in this
     # run; A0 = -1 on every unit-scaled row of this system, so 1/A0 is not read)
     kern_avg_s = prof["total_ms"] * 1e-3 / max(1, prof["launches"])
@@ -661,8 +661,12 @@ def main(argv=None):
     traffic = None
     tf = ROOT / "profiles" / "stencil_fp32_256cubed_traffic.json"
     if tf.exists():
+        # measured HBM bytes per launch of the shape this run used (autotuned
+        # shapes differ in their halo re-reads), else of the counter runs' shape
         try:
-            traffic = json.loads(tf.read_text()).get("bytes_per_launch")
+            tj = json.loads(tf.read_text())
+            cells = shapes.get("mixed", "256x1").split("x")[0]
+            traffic = tj.get("bytes_per_launch_by_shape", {}).get(cells, tj.get("bytes_per_launch"))
         except Exception:
             traffic = None
     # whole-solve algorithmic rate: iterations x fused-schedule bytes per cell
@@ -710,7 +714,7 @@ def main(argv=None):
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_matvec_dots<float, 0> (FP32 stencil V = A*(P/A0) + <Rtilde,V>)",
+            "kernel": "k_matvec_dots2<float, 0> (FP32 stencil V = A*(P/A0) + <Rtilde,V>)",
             "achieved": achieved / 1e9,
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",

@@ -244,6 +244,65 @@ __global__ void __launch_bounds__(BS) k_matvec_dots(const Sys<T> *__restrict__ s
     }
 }
 
+// k_matvec_dots with two cells per thread, BS/2 threads over the same BS-cell
+// logical blocks: thread t takes cells t and t + BS/2 and issues the loads of
+// both rows before it waits on either, so a wave keeps twice the loads in
+// flight for one pass through the block's set-up.  FP32 inner solve only (a
+// thread sums its two cells before the wave tree, so the partials round
+// differently from the one-cell kernel).
+template <int BS, typename T, int MODE>
+__global__ void __launch_bounds__(BS / 2) k_matvec_dots2(const Sys<T> *__restrict__ sys,
+                                                          const int2 *__restrict__ map) {
+    constexpr int NT = BS / 2;
+    __shared__ double lds[(NT / 64) * 2];
+    __shared__ unsigned arrived;
+    SYS_BLOCK(T)
+    auto *__restrict__ st = to_global(S.st);
+    if (st->done) return;
+    if (MODE == 1 && st->s_exit) return;
+    partial_begin<NT>(&arrived);
+    double v[2] = {0.0, 0.0};
+    const bool all = MODE == 1 ? (g.dot_all && !st->minimize) : g.dot_all;
+    auto *__restrict__ a = to_global(S.a);
+    const auto a0 = a + (int64_t)DiagPlane<T>::value * g.n3;
+    auto *__restrict__ y = MODE == 0 ? to_global(S.v) : to_global(S.t);
+    auto *__restrict__ w = MODE == 0 ? to_global(S.rt) : to_global(S.s);
+    const auto xin = MODE == 0 ? (S.pc == PC_LINE ? to_global(S.phat) : to_global(S.p)) : (S.pc == PC_LINE ? to_global(S.shat) : to_global(S.s));
+    const int64_t lo = blk * P.chunk, end = lo + P.chunk < P.n ? lo + P.chunk : P.n;
+    auto body = [&](auto X) {
+      for (int64_t base = lo; base < end; base += BS) {
+        const int64_t ca = base + threadIdx.x, cq = ca + NT;
+        const bool va = ca < end, vb = cq < end;
+        const int64_t c0 = va ? ca : base, c1 = vb ? cq : c0;  // in bounds even when not stored
+        const Nb n0 = neighbours_fast(g, c0), n1 = neighbours_fast(g, c1);
+        const T w0 = ld_stream(&w[c0]), w1 = ld_stream(&w[c1]);
+        const T y0 = Stencil<T>::row(g, a, X, c0, n0);
+        const T y1 = Stencil<T>::row(g, a, X, c1, n1);
+        if (va) y[c0] = y0;
+        if (vb) y[c1] = y1;
+        if (va && (all || n0.interior)) {
+            const double yd = (double)y0, wd = (double)w0;
+            v[0] += (MODE == 0) ? wd * yd : yd * wd;
+            if (MODE == 1) v[1] += yd * yd;
+        }
+        if (vb && (all || n1.interior)) {
+            const double yd = (double)y1, wd = (double)w1;
+            v[0] += (MODE == 0) ? wd * yd : yd * wd;
+            if (MODE == 1) v[1] += yd * yd;
+        }
+      }
+    };
+    if (S.pc == PC_DIAG && st->nonunit_diag) body(Operand<T, OP_DIAG, gptr<const T>>{xin, a0});
+    else if (S.pc == PC_DIAG) body(Operand<T, OP_NEG, gptr<const T>>{xin, nullptr});
+    else body(Operand<T, OP_PLAIN, gptr<const T>>{xin, nullptr});
+    if (MODE == 0) {
+        double v1[1] = {v[0]};
+        block_partial_last<NT, 1>(v1, to_global(S.partials), P.nblk, blk, lds, &arrived);
+    } else {
+        block_partial_last<NT, 2>(v, to_global(S.partials), P.nblk, blk, lds, &arrived);
+    }
+}
+
 // S = R - alpha*V (leq_bicgs.f:365-367) and ||S||^2 (:373); for LINE also
 // the preconditioner's start value Shat = S.  The S-norm exit test is applied
 // by the finish after the second stencil pass.

@@ -575,12 +575,18 @@ static void iteration_bs(const Group<T> &G, const MapSet &M, hipStream_t s) {
     hipLaunchKernelGGL((k_p_update<BS, T>), gs, gblk, 0, s, G.d_sys, M.d_smap);
     if (line) line_sweeps(G, M, 0, s);
     pf.begin(s);
-    hipLaunchKernelGGL((k_matvec_dots<BS, T, 0>), gs, blk, 0, s, G.d_sys, M.d_smap);
+    if constexpr (sizeof(T) == sizeof(float))
+        hipLaunchKernelGGL((k_matvec_dots2<BS, T, 0>), gs, dim3(BS / 2), 0, s, G.d_sys, M.d_smap);
+    else
+        hipLaunchKernelGGL((k_matvec_dots<BS, T, 0>), gs, blk, 0, s, G.d_sys, M.d_smap);
     pf.end(s, M.k1_bytes);
     finish<T, FIN_ALPHA, 1>(G, M, s);
     hipLaunchKernelGGL((k_s_update<BS, T>), gs, gblk, 0, s, G.d_sys, M.d_smap);
     if (line) line_sweeps(G, M, 1, s);
-    hipLaunchKernelGGL((k_matvec_dots<BS, T, 1>), gs, blk, 0, s, G.d_sys, M.d_smap);
+    if constexpr (sizeof(T) == sizeof(float))
+        hipLaunchKernelGGL((k_matvec_dots2<BS, T, 1>), gs, dim3(BS / 2), 0, s, G.d_sys, M.d_smap);
+    else
+        hipLaunchKernelGGL((k_matvec_dots<BS, T, 1>), gs, blk, 0, s, G.d_sys, M.d_smap);
     finish<T, FIN_OMEGA, 3>(G, M, s);
     hipLaunchKernelGGL((k_xr_update<BS, T>), gs, gblk, 0, s, G.d_sys, M.d_smap);
     if (G.mixed) {
