@@ -466,25 +466,32 @@ __device__ __forceinline__ bool finish_sum(PP partials, int64_t nblk, int fb, in
     const int64_t per = (nblk + nfb - 1) / nfb;
     const int64_t lo = (int64_t)fb * per;
     const int64_t hi = lo + per < nblk ? lo + per : nblk;
+    // the NV sums advance together, so their loads share one round trip;
+    // each still adds its partials in block order
 #pragma unroll
-    for (int q = 0; q < NV; ++q) {
-        const auto p = partials + q * nblk;
-        double s = 0.0;
-        for (int64_t b = lo + threadIdx.x; b < hi; b += kFinishBlock) s += p[b];
-        tot[q] = s;
+    for (int q = 0; q < NV; ++q) tot[q] = 0.0;
+    for (int64_t b = lo + threadIdx.x; b < hi; b += kFinishBlock) {
+        double pv[NV];
+#pragma unroll
+        for (int q = 0; q < NV; ++q) pv[q] = partials[q * nblk + b];
+#pragma unroll
+        for (int q = 0; q < NV; ++q) tot[q] += pv[q];
     }
     block_sum<kFinishBlock, NV>(tot, lds);
     if (nfb == 1) return true;
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int q = 0; q < NV; ++q) l2[q * nfb + fb] = tot[q];
-        __threadfence();
-        const unsigned prev = atomicAdd((unsigned *)counter, 1u);
+        // release: this block's level-2 sums are visible before its count
+        const unsigned prev = __hip_atomic_fetch_add((unsigned *)counter, 1u, __ATOMIC_RELEASE,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
         last = prev == (unsigned)nfb - 1;
     }
     __syncthreads();
     if (!last) return false;
-    __threadfence();
+    // acquire: every other block's sums, released before its count, are read
+    // after the count that made this block the last
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 #pragma unroll
     for (int q = 0; q < NV; ++q) tot[q] = threadIdx.x < nfb ? load_agent(&l2[q * nfb + threadIdx.x]) : 0.0;
     __syncthreads();
