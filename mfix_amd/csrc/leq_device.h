@@ -310,6 +310,11 @@ __host__ __device__ __forceinline__ int64_t update_blocks(int64_t nblk) {
     const int64_t r = (nblk + kNumXcd - 1) / kNumXcd * kNumXcd;
     return r < kUpdateBlocks ? r : kUpdateBlocks;
 }
+// Partial-sum slots per array of a system: one per logical block of the
+// streaming kernels, and at least one per block of the reliable-update launch.
+__host__ __device__ __forceinline__ int64_t partial_slots(int64_t nblk) {
+    return nblk > kUpdateBlocks ? nblk : (int64_t)kUpdateBlocks;
+}
 
 __device__ __forceinline__ int64_t xcd_block(int64_t nblocks) {
     const int64_t b = blockIdx.x;

@@ -52,6 +52,7 @@ struct SolverState {
     int restart;            // mixed: restart the recurrence (p = r, Rtilde = r) at the next iteration
     int restarts;           // mixed: recurrence restarts (one per reliable update that did not end the solve)
     unsigned fin_counter;   // arrivals of the current finish launch (reset by its last block)
+    unsigned upd_counter;   // mixed: arrivals of the current reliable-update launch (reset by its last block)
     int nonunit_diag;       // some unit-scaled A0 differs from -1 (DIAG must read the A0 plane)
     double l2[4 * kFinishBlocks];  // level-2 sums of the finish launches
 };

@@ -461,8 +461,11 @@ __global__ void __launch_bounds__(BS / Group16<T>::N) k_xr_update(const Sys<T> *
 // The kernel is launched every iteration but works only every ~100, so each
 // system gets at most kUpdateBlocks blocks (map entry y = 0..ub-1) that loop
 // over the logical blocks of their XCD's slab: an idle launch retires in a
-// few microseconds.  Each logical block's partials are summed exactly as a
-// one-block-per-chunk launch would sum them.
+// few microseconds.  Each block keeps one partial per sum over all its
+// logical blocks (a fixed order for a given system size) and counts itself
+// in with a release increment; the last block to arrive sums the ub
+// partials in block order and applies the update's scalar logic (FIN_UPDATE)
+// itself, so no finish launch follows the (mostly idle) update.
 template <int BS>
 __global__ void __launch_bounds__(BS) k_reliable_update(const Sys<float> *__restrict__ sys,
                                                         const int2 *__restrict__ map) {
@@ -489,8 +492,8 @@ __global__ void __launch_bounds__(BS) k_reliable_update(const Sys<float> *__rest
     const int64_t ub = update_blocks(P.nblk), nper = ub / kNumXcd;
     const int64_t per = (P.nblk + kNumXcd - 1) / kNumXcd;
     const int64_t lo = (e_.y % kNumXcd) * per, hi = lo + per < P.nblk ? lo + per : P.nblk;
+    double acc[3] = {0.0, 0.0, 0.0};
     for (int64_t blk = lo + e_.y / kNumXcd; blk < hi; blk += nper) {
-        double acc[3] = {0.0, 0.0, 0.0};
         FOR_CELLS(BS, P, blk, c) {
             const Nb n = neighbours_fast(g, c);
             const float rtc = rt[c];
@@ -505,8 +508,30 @@ __global__ void __launch_bounds__(BS) k_reliable_update(const Sys<float> *__rest
                 acc[2] += (double)r32 * (double)r32;
             }
         }
-        block_partial<BS, 3>(acc, to_global(S.partials), P.nblk, blk, lds);
-        __syncthreads();  // lds is reused by the next logical block
+    }
+    auto *__restrict__ part = to_global(S.partials);  // 3 x ub slots (partial_slots)
+    __shared__ bool last;
+    block_sum<BS, 3>(acc, lds);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) part[q * ub + e_.y] = acc[q];
+        const unsigned prev = __hip_atomic_fetch_add(&st->upd_counter, 1u, __ATOMIC_RELEASE,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
+        last = prev == (unsigned)(ub - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    double tot[3] = {0.0, 0.0, 0.0};
+    for (int64_t u = threadIdx.x; u < ub; u += BS) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) tot[q] += load_agent(&part[q * ub + u]);
+    }
+    __syncthreads();  // lds is reused
+    block_sum<BS, 3>(tot, lds);
+    if (threadIdx.x == 0) {
+        st->upd_counter = 0u;
+        fin_apply<FIN_UPDATE, 3>(st, tot, nullptr);
     }
 }
 

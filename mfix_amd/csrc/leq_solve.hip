@@ -319,7 +319,7 @@ size_t sys_bytes(const DGrid &g, const Part &P, int pc, bool mixed) {
     if (mixed) b += align256(n3 * sizeof(double));  // y
     if (line) b += 2 * align256(n3 * sv) + align256((size_t)line_scratch_elems(g) * sv);
     if (mixed) b += align256(8 * n3 * sizeof(float)) + 2 * align256(n3 * sizeof(double));
-    b += align256(sizeof(double) * kMaxPartials * (size_t)P.nblk);
+    b += align256(sizeof(double) * kMaxPartials * (size_t)partial_slots(P.nblk));
     b += align256(sizeof(SolverState));
     return b;
 }
@@ -374,7 +374,7 @@ void sys_carve(Sys<T> &S, const SysDesc &d, const Part &P, char *base) {
         S.r64 = (double *)S.r;
         S.xb = nullptr;
     }
-    S.partials = (double *)take(sizeof(double) * kMaxPartials * (size_t)P.nblk);
+    S.partials = (double *)take(sizeof(double) * kMaxPartials * (size_t)partial_slots(P.nblk));
     S.st = (SolverState *)take(sizeof(SolverState));
     S.a64 = d.a;
     S.b64 = d.b;
@@ -591,9 +591,9 @@ static void iteration_bs(const Group<T> &G, const MapSet &M, hipStream_t s) {
     hipLaunchKernelGGL((k_xr_update<BS, T>), gs, gblk, 0, s, G.d_sys, M.d_smap);
     if (G.mixed) {
         finish<T, FIN_XR_MIXED, 2>(G, M, s);
+        // (its last block applies the update's scalar logic, FIN_UPDATE)
         hipLaunchKernelGGL((k_reliable_update<BS>), dim3((unsigned)M.umap.size()), blk, 0, s,
                            (const Sys<float> *)G.d_sys, M.d_umap);
-        finish<T, FIN_UPDATE, 3>(G, M, s);
     } else {
         finish<T, FIN_XR, 2>(G, M, s);
     }
