@@ -140,7 +140,11 @@ int mfix_leq_dot_dev(const double *d_x, const double *d_y, double *result,
                      void *stream);
 /* Row max-norm unit scaling, leq_bicgs.f:194-233; *ier = -2 on a zero row. */
 int mfix_leq_scale_dev(double *d_a_m, double *d_b_m, int *ier, void *stream);
-/* FP32 variants used by the mixed-precision inner solver. */
+/* FP32 stencil row of the mixed-precision inner solver (no reference
+ * counterpart): d_a_m holds 7 FP32 planes in the difference form, planes
+ * 0-2 and 4-6 the off-diagonal couplings as in A_M and plane 3 the row
+ * excess A0 + sum of the six couplings; y = sum_l a_l (x_nb - x_c) + excess
+ * x_c, products in FP32, sums in FP64, one rounding to FP32. */
 int mfix_leq_matvec32_dev(const float *d_a_m, const float *d_x, float *d_y,
                           void *stream);
 

@@ -872,7 +872,7 @@ __global__ void __launch_bounds__(BS) k_matvec(DGrid g, Part P, const T *__restr
     const Operand<T, OP_PLAIN> X{x, nullptr};
     FOR_CELLS(BS, P, blk, c) {
         const Nb n = neighbours_fast(g, c);
-        y[c] = stencil_row<T>(g, a, X, c, n);
+        y[c] = Stencil<T>::row(g, a, X, c, n);  // FP32: the inner solve's difference form
     }
 }
 
