@@ -193,6 +193,9 @@ def bicgs_dev(var: DeviceArray, a_m: DeviceArray, b_m: DeviceArray, *, pc: str =
 
 
 def matvec_dev(a_m: DeviceArray, x: DeviceArray, y: DeviceArray) -> None:
+    """y = A x.  FP64: LEQ_MATVEC on A_M in the reference layout
+    (model/leqsol_mod.f:103-265).  FP32: the mixed inner solve's row on the
+    difference form (plane 3 holds the row excess A0 + sum of the couplings)."""
     if a_m.dtype == np.float32:
         check(_lib_().mfix_leq_matvec32_dev(a_m.ptr, x.ptr, y.ptr, None), "matvec32")
     else:
