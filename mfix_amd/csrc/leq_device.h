@@ -451,12 +451,12 @@ __device__ __forceinline__ void block_partial_last(double (&v)[NV], PD partials,
 // Finish step, run by nfb <= kFinishBlocks blocks of kFinishBlock threads
 // (fb = this block's index among them).  Block fb sums the partials of
 // slice fb in a fixed order into level-2 sums; the
-// last block to arrive (counted with one device-scope atomic per block)
-// adds the level-2 sums in slice order, so the result does not depend on
-// arrival order.  Only the few level-2 values are written before the
-// release fence, so the fence has almost nothing to write back.  Returns
-// true in the last block, with the totals in thread 0; the counter is
-// reset there for the next launch.
+// last block to arrive (counted with one device-scope release increment
+// per block; the last takes an acquire fence) adds the level-2 sums in slice
+// order, so the result does not depend on arrival order.  Only the few
+// level-2 values are written before each release, so it has almost nothing
+// to write back.  Returns true in the last block, with the totals in thread
+// 0; the counter is reset there for the next launch.
 constexpr int kFinishBlocks = 64;
 
 template <typename P>
