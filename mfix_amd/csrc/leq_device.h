@@ -389,8 +389,8 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double *lds) {
 }
 
 // Grid-wide sums in two steps.  Each block stores its NV partial sums with
-// plain stores; a one-block finish kernel launched next sums them in
-// block-index order.  The kernel boundary makes the partials visible across
+// plain stores; the finish kernel launched next sums them in a fixed order
+// (finish_sum below).  The kernel boundary makes the partials visible across
 // the eight XCDs' L2s once per launch, where a per-block release fence
 // (last-block-done pattern) would write back each XCD's L2 for every block.
 template <int BS, int NV, typename PD>
