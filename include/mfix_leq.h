@@ -407,12 +407,16 @@ int mfix_leq_set_graphs(int on);
 int mfix_leq_set_unit_diag(int on);
 
 /* ---- timing of the dominant kernel (HIP events on the solver stream) ---- */
-/* Enable per-launch event timing of kernel `which` (0 = fused stencil
- * kernel, 1 = line preconditioner kernel); read back the launch count, the
- * summed milliseconds and the algorithmic bytes those launches moved. */
+/* Enable per-launch event timing of kernel `which` (0 = first stencil pass
+ * of each iteration, 1 = line preconditioner colour phase); read back the
+ * launch count, the summed milliseconds and the algorithmic bytes of the
+ * launches that worked.  Launches of a single solve issued after it was
+ * done (they return at once) are counted apart: mfix_leq_prof_idle.
+ * No counterpart in the reference (measurement of this library). */
 int mfix_leq_prof_enable(int which, int on);
 int mfix_leq_prof_read(int which, int64_t *launches, double *total_ms,
                        double *alg_bytes);
+int mfix_leq_prof_idle(int which, int64_t *launches, double *total_ms);
 
 #ifdef __cplusplus
 }

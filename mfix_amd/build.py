@@ -93,7 +93,7 @@ def build_fortran(verbose: bool = False) -> Path | None:
     objs = []
     for src in srcs:
         obj = objdir / (src.stem + ".o")
-        cmd = [fc, "-O2", "-fPIC", "-c", str(src), "-o", str(obj), "-module-dir", str(objdir)]
+        cmd = [fc, "-cpp", "-O2", "-fPIC", "-c", str(src), "-o", str(obj), "-module-dir", str(objdir)]
         if verbose:
             print(" ".join(cmd))
         _run(cmd)

@@ -248,8 +248,13 @@ int mfix_leq_msolve_dev(const char *pc, const char *cmethod, const double *d_a_m
     int *flag = (int *)C.misc.ptr;
     e = hipMemsetAsync(flag, 0, sizeof(int), s);
     if (e != hipSuccess) return hip_fail(e, "memset");
-    launch_msolve_line<double>(g, C.shape_for(g.n3, false), cmethod ? cmethod : "RSRS", d_a_m, d_r, d_z,
-                               (double *)C.tmp.ptr, nullptr, 0, flag, s, true);
+    // R/S sweep strings: the solver's own red-black kernel; others the
+    // per-line phases
+    rc = run_msolve_line_rb(g, cmethod ? cmethod : "RSRS", d_a_m, d_r, d_z, (double *)C.tmp.ptr, s);
+    if (rc < 0) return rc;
+    if (rc == 1)
+        launch_msolve_line<double>(g, C.shape_for(g.n3, false), cmethod ? cmethod : "RSRS", d_a_m, d_r, d_z,
+                                   (double *)C.tmp.ptr, nullptr, 0, flag, s, true);
     int hflag = 0;
     e = hipMemcpyAsync(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost, s);
     if (e != hipSuccess) return hip_fail(e, "copy");
@@ -318,7 +323,12 @@ int mfix_solve_lin_eq_deferred(const char *vname, int vname_len, int vno, double
     d.item.jmax = C.grid.jmax;
     d.item.kmax = C.grid.no_k ? 1 : C.grid.kmax;
     std::memcpy(d.item.pc, pc, 4);
-    for (int i = 0; i < 4; ++i) d.item.cmethod[i] = sweep[i] ? sweep[i] : ' ';
+    // a C string shorter than 4 characters is blank-padded, never read past its NUL
+    bool ended = false;
+    for (int i = 0; i < 4; ++i) {
+        ended = ended || sweep[i] == '\0';
+        d.item.cmethod[i] = ended ? ' ' : sweep[i];
+    }
     d.item.tol = tol1;
     d.item.itmax = itmax;
     d.item.mixed = mixed != 0;
@@ -388,6 +398,16 @@ int mfix_leq_prof_enable(int which, int on) {
     p.launches = 0;
     p.total_ms = 0.0;
     p.alg_bytes = 0.0;
+    p.idle_launches = 0;
+    p.idle_ms = 0.0;
+    return 0;
+}
+
+int mfix_leq_prof_idle(int which, int64_t *launches, double *total_ms) {
+    if (which < 0 || which >= kNumProf) return MFIX_LEQ_EINVAL;
+    const Prof &p = ctx().prof[which];
+    if (launches) *launches = p.idle_launches;
+    if (total_ms) *total_ms = p.idle_ms;
     return 0;
 }
 

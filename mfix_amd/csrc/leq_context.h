@@ -23,17 +23,22 @@ struct DevBuf {
 };
 
 // Event timing of one kernel family (0: stencil kernels, 1: line phases).
+// Launches after the solve was done return at once; collect(working) counts
+// only the first `working` pending launches (in stream order) as work and
+// the rest as idle, so the byte rate is over launches that did the work.
 struct Prof {
     bool on = false;
     int64_t launches = 0;
     double total_ms = 0.0;
     double alg_bytes = 0.0;
+    int64_t idle_launches = 0;
+    double idle_ms = 0.0;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
     std::vector<double> pending_bytes;
     size_t pool_used = 0;
     void begin(hipStream_t s);
     void end(hipStream_t s, double bytes);
-    void collect();
+    void collect(int64_t working = -1);
 };
 constexpr int kNumProf = 2;
 constexpr int kHostStates = 1024; // pinned state mirrors (max systems read back at once)
@@ -96,6 +101,7 @@ struct Context {
     mfix_leq_grid grid = {};
     int iter_tot[10] = {0};
     DevBuf d_a, d_b, d_x;          // host-entry staging of A_M, B_M, VAR
+    DevBuf line1;                  // one-system table, state and block map of a standalone LINE application
     DevBuf ws;                     // single-solve workspace (carved per solve)
     DevBuf bws;                    // batch workspace
     DevBuf partials, state, misc, tmp;
@@ -178,6 +184,10 @@ int run_diag(const DGrid &g, const double *a, const double *r, double *z, hipStr
 int run_dot(const DGrid &g, const double *x, const double *y, double *d_out, hipStream_t s);
 int run_scale(const DGrid &g, double *a, double *b, int *zero_row, hipStream_t s);
 int run_copy(const double *src, double *dst, int64_t n, hipStream_t s);
+// LEQ_MSOLVE (LINE) with a sweep string of R/S passes through the batched
+// red-black kernel; returns 1 (nothing launched) for other sweep strings.
+int run_msolve_line_rb(const DGrid &g, const char *cmethod, const double *a, const double *r, double *z,
+                       double *scratch, hipStream_t s);
 int autotune(int imax, int jmax, int kmax, int mixed, Shape *best);
 // CALC_RESID_PP of device arrays on the current grid (leq_resid.hip).
 int resid_pp(const double *b, const int *flag, double norm, int debug, mfix_leq_resid *out, hipStream_t s);

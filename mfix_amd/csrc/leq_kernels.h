@@ -54,6 +54,8 @@ struct SolverState {
     unsigned fin_counter;   // arrivals of the current finish launch (reset by its last block)
     unsigned upd_counter;   // mixed: arrivals of the current reliable-update launch (reset by its last block)
     int nonunit_diag;       // some unit-scaled A0 differs from -1 (DIAG must read the A0 plane)
+    int k1_launches;        // launches of the first stencil pass that worked on this system (timing)
+    int line_launches;      // colour-phase launches that worked on this system (timing)
     double l2[4 * kFinishBlocks];  // level-2 sums of the finish launches
 };
 

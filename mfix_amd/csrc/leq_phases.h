@@ -211,6 +211,7 @@ __global__ void __launch_bounds__(BS) k_matvec_dots(const Sys<T> *__restrict__ s
     auto *__restrict__ st = to_global(S.st);
     if (st->done) return;
     if (MODE == 1 && st->s_exit) return;
+    if (MODE == 0 && blk == 0 && threadIdx.x == 0) st->k1_launches += 1;  // (timing: working launches)
     partial_begin<BS>(&arrived);
     double v[2] = {0.0, 0.0};
     const bool all = MODE == 1 ? (g.dot_all && !st->minimize) : g.dot_all;
@@ -260,6 +261,7 @@ __global__ void __launch_bounds__(BS / 2) k_matvec_dots2(const Sys<T> *__restric
     auto *__restrict__ st = to_global(S.st);
     if (st->done) return;
     if (MODE == 1 && st->s_exit) return;
+    if (MODE == 0 && blk == 0 && threadIdx.x == 0) st->k1_launches += 1;  // (timing: working launches)
     partial_begin<NT>(&arrived);
     double v[2] = {0.0, 0.0};
     const bool all = MODE == 1 ? (g.dot_all && !st->minimize) : g.dot_all;
@@ -616,6 +618,7 @@ __global__ void __launch_bounds__(192) k_line_color_sys(const Sys<T> *__restrict
     const Sys<T> &S = sys[e.x];
     auto *__restrict__ st = to_global(S.st);
     if (st->done || (which == 1 && st->s_exit)) return;
+    if (e.y == 0 && threadIdx.x == 0) st->line_launches += 1;  // (timing: working launches)
     const DGrid &g = S.g;
     const LineGeom L = line_geom(g, AX_J);
     const int n = L.n;

@@ -115,7 +115,7 @@ int Context::init(int dev) {
 }
 
 void Context::release_all() {
-    for (DevBuf *b : {&d_a, &d_b, &d_x, &ws, &bws, &partials, &state, &misc, &tmp}) b->release();
+    for (DevBuf *b : {&d_a, &d_b, &d_x, &ws, &bws, &partials, &state, &misc, &tmp, &line1}) b->release();
     g64.drop_graph();
     g32.drop_graph();
     b64.drop_graph();
@@ -157,11 +157,16 @@ void Prof::end(hipStream_t s, double bytes) {
     ++pool_used;
 }
 
-void Prof::collect() {
+void Prof::collect(int64_t working) {
     for (size_t q = 0; q < pool_used; ++q) {
         float ms = 0.f;
         if (hipEventSynchronize(events[q].second) == hipSuccess &&
             hipEventElapsedTime(&ms, events[q].first, events[q].second) == hipSuccess) {
+            if (working >= 0 && (int64_t)q >= working) {
+                idle_ms += ms;
+                ++idle_launches;
+                continue;
+            }
             total_ms += ms;
             alg_bytes += pending_bytes[q];
             ++launches;
@@ -198,6 +203,7 @@ int size_class(int64_t n3) {
     }
 
 static inline dim3 grid_of(const Part &P) { return dim3((unsigned)xcd_grid(P.nblk)); }
+static inline size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
 static inline int finish_blocks(int64_t nblk) {
     // about one partial per finish thread, at most kFinishBlocks blocks:
@@ -289,6 +295,61 @@ int launch_msolve_line(const DGrid &g, const Shape &sh, const char *cmethod, con
     return 0;
 }
 
+// LEQ_MSOLVE (leqsol_mod.f:286-598) with a sweep string whose characters are
+// R passes or characters that do nothing here (S; J; K and A in 2-D),
+// through the red-black kernel the solver uses (k_line_color_sys): z = r
+// (SETGUESS, :341-357), then per R pass the (i+k)-odd colour and the even
+// one.  A one-system table carries the arrays; its state is blank, so the
+// kernel never returns early.  Returns 1, launching nothing, for a sweep
+// string with I/K/A passes (the caller runs k_line_phase instead).
+int run_msolve_line_rb(const DGrid &g, const char *cmethod, const double *a, const double *r, double *z,
+                       double *scratch, hipStream_t s) {
+    Sweep sw = {'R', 'S', 'R', 'S'};
+    if (cmethod)
+        for (int q = 0; q < 4 && cmethod[q]; ++q) sw[q] = cmethod[q];
+    int np = 0;
+    for (char ch : sw) {
+        if (ch == 'R' || ch == 'r') ++np;
+        else if (ch == 'I' || ch == 'i') return 1;
+        else if (g.do_k && (ch == 'K' || ch == 'k' || ch == 'A' || ch == 'a')) return 1;
+    }
+    Context &C = ctx();
+    const LineGeom L = line_geom(g, AX_J);
+    const int64_t lines = ((int64_t)L.nu * L.nv + 1) / 2;
+    const int64_t lb = (lines + 63) / 64;
+    const size_t o_st = align256(sizeof(Sys<double>));
+    const size_t o_map = o_st + align256(sizeof(SolverState));
+    int rc;
+    if ((rc = C.line1.ensure(o_map + sizeof(int2) * (size_t)lb))) return rc;
+    char *base = (char *)C.line1.ptr;
+    Sys<double> S = {};
+    S.g = g;
+    S.pc = PC_LINE;
+    S.a = a;
+    S.p = const_cast<double *>(r);
+    S.phat = z;
+    S.line_scratch = scratch;
+    S.st = (SolverState *)(base + o_st);
+    std::vector<int2> map((size_t)lb);
+    for (int64_t b = 0; b < lb; ++b) map[(size_t)b] = make_int2(0, (int)b);
+    hipError_t e = hipMemcpyAsync(base, &S, sizeof(S), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(base + o_map, map.data(), sizeof(int2) * map.size(), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return hip_fail(e, "line table upload");
+    hipLaunchKernelGGL(k_init_state, dim3(1), dim3(1), 0, s, S.st);
+    run_copy(r, z, g.n3, s);
+    Prof &pf = C.prof[1];
+    for (int q = 0; q < np; ++q)
+        for (int param : {1, 0}) {
+            pf.begin(s);
+            hipLaunchKernelGGL(k_line_color_sys<double>, dim3((unsigned)lb), dim3(192), 0, s,
+                               (const Sys<double> *)base, (const int2 *)(base + o_map), param, 0);
+            pf.end(s, 0.5 * (double)g.n3 * 13.0 * sizeof(double));
+        }
+    // the table must outlive the launches before it is rewritten
+    e = hipStreamSynchronize(s);
+    return e == hipSuccess ? 0 : hip_fail(e, "line solve");
+}
+
 template int launch_msolve_line<double>(const DGrid &, const Shape &, const char *, const double *, const double *,
                                         double *, double *, const SolverState *, int, int *, hipStream_t, bool);
 template int launch_msolve_line<float>(const DGrid &, const Shape &, const char *, const float *, const float *,
@@ -302,8 +363,6 @@ double alg_bytes_per_cell(int pc, int sa, int sv, int sx) {
 }
 
 // ---- system workspaces --------------------------------------------------------
-
-static inline size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
 // FP64: R, Rtilde, P, S, T, V (+ Phat, Shat and the DGTSV scratch for LINE).
 // Mixed: the FP32 r, rt, p, s, t, v (+ LINE arrays), the FP64 correction
@@ -753,6 +812,7 @@ void fill_stats(const SolverState &h, mfix_leq_stats *st) {
     st->ier = h.ier;
     st->exit_reason = h.exit_reason;
     if (h.exit_reason == EXIT_ZERO_ROW) st->iterations = 0;
+    else if (h.mixed) st->iterations = h.iter_var - 1;  // inner iterations completed (FIN_XR_MIXED counts each)
     else if (h.exit_reason == EXIT_RHO) st->iterations = h.i - 1;
     else st->iterations = h.i;
     st->iter_tot_increment = h.iter_tot_inc;
@@ -800,10 +860,15 @@ static int solve_one(Group<T> &G, const SysDesc &d, const Shape &sh, mfix_leq_st
     (void)hipEventElapsedTime(&ms, e0, e1);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    for (auto &p : C.prof) p.collect();
-    if (rc) return rc;
     SolverState *h = C.host_state;
-    if ((rc = read_states<T>(G, std::vector<int>{0}, h, s))) return rc;
+    if (rc || (rc = read_states<T>(G, std::vector<int>{0}, h, s))) {
+        for (auto &p : C.prof) p.collect();
+        return rc;
+    }
+    // a one-system table: its first launches did the work, later ones
+    // returned at once (the solve was done)
+    C.prof[0].collect(h->k1_launches);
+    C.prof[1].collect(h->line_launches);
     if (mixed && h->x_cur == 1) {
         hipError_t e = hipMemcpyAsync(d.x, G.sys[0].xb, sizeof(double) * d.g.n3, hipMemcpyDeviceToDevice, s);
         if (e == hipSuccess) e = hipStreamSynchronize(s);

@@ -8,6 +8,12 @@
 ! (include/mfix_leq.h).  MFIX_LEQ_GRID_INIT replaces the module state the
 ! reference routine reads (compar/geometry/indices, set by GRIDMAP_INIT and
 ! SET_INCREMENTS) and must be called once after the grid is known.
+!
+! Built inside the MFIX tree with -DMFIX_LEQSOL_ITER_TOT (preprocessed), the
+! drop-in also adds each solve's count to leqsol's ITER_TOT(VNO), as
+! LEQ_BICGS0 does (model/leq_bicgs.f:458, 491), so REPORT_SOLVER_STATS
+! (model/leqsol_mod.f:50-84) prints the GPU solves' iterations.  Built on its
+! own, the library keeps the counts (mfix_leq_iter_tot_f).
 module mfix_leq_c
   use iso_c_binding
   implicit none
@@ -70,6 +76,9 @@ end subroutine mfix_leq_grid_init_f
 ! Drop-in LEQ_BICGS: model/leq_bicgs.f:23-69 argument list.
 subroutine leq_bicgs(vname, vno, var, a_m, b_m, cmethod, tol, pc, itmax, ier)
   use mfix_leq_c
+#ifdef MFIX_LEQSOL_ITER_TOT
+  use leqsol, only: iter_tot
+#endif
   implicit none
   character(len=*), intent(in) :: vname
   integer, intent(in) :: vno
@@ -82,6 +91,10 @@ subroutine leq_bicgs(vname, vno, var, a_m, b_m, cmethod, tol, pc, itmax, ier)
   integer, intent(in) :: itmax
   integer, intent(inout) :: ier
   integer(c_int) :: rc, cier
+#ifdef MFIX_LEQSOL_ITER_TOT
+  integer(c_int) :: before
+  before = c_iter_tot(vno)
+#endif
   rc = c_bicgs(vname, len(vname, kind=c_int), vno, var, a_m, b_m, cmethod, len(cmethod, kind=c_int), tol, pc, &
                itmax, cier)
   if (rc == MFIX_LEQ_EBADPC) then
@@ -97,6 +110,9 @@ subroutine leq_bicgs(vname, vno, var, a_m, b_m, cmethod, tol, pc, itmax, ier)
      error stop 1
   end if
   ier = cier
+#ifdef MFIX_LEQSOL_ITER_TOT
+  iter_tot(vno) = iter_tot(vno) + (c_iter_tot(vno) - before)
+#endif
 end subroutine leq_bicgs
 
 ! ITER_TOT(VNO) as kept by the reference (leqsol_mod.f:17).

@@ -231,7 +231,11 @@ def prof_read(which: int) -> dict:
     ms = C.c_double()
     by = C.c_double()
     check(_lib_().mfix_leq_prof_read(which, C.byref(n), C.byref(ms), C.byref(by)), "prof_read")
-    return {"launches": n.value, "total_ms": ms.value, "alg_bytes": by.value}
+    ni = C.c_int64()
+    mi = C.c_double()
+    check(_lib_().mfix_leq_prof_idle(which, C.byref(ni), C.byref(mi)), "prof_idle")
+    return {"launches": n.value, "total_ms": ms.value, "alg_bytes": by.value, "idle_launches": ni.value,
+            "idle_ms": mi.value}
 
 
 def set_graphs(on: bool) -> None:

@@ -20,12 +20,48 @@
  * Pinning (see tests/test_oracle_*.py): golden vectors written by the
  * compiled reference (tests/golden/), scipy's LAPACK dgtsv, brute-force
  * numpy stencils, a sparse direct solve, and closed-form special cases.
+ *
+ * Threading follows the reference's OpenMP structure, so the timed CPU
+ * baseline is the reference's parallel algorithm and not a serial one:
+ * LEQ_MATVEC is a parallel loop over cells (leqsol_mod.f:188-189), the row
+ * scaling a parallel loop over rows (leq_bicgs.f:197-226), the vector
+ * updates parallel workshares (leq_bicgs.f:285-436), DIAG a parallel loop
+ * (leqsol_mod.f:730), and each red-black pass of LINE a parallel loop over
+ * the lines of one colour (leqsol_mod.f:497-522, 2-D :383-391).  The dot
+ * products stay single-threaded in storage order: with IS_SERIAL (the
+ * default) the reference calls the serial intrinsic dot_product
+ * (leqsol_mod.f:1125-1136).  Every parallel loop is element-wise or over
+ * independent lines, so the results are the same bits at any thread count.
  */
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
 #include <stdio.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* Threads of the parallel loops (0 or less: leave the OpenMP default);
+ * returns the count in effect before the call. */
+int oracle_set_threads(int n) {
+#ifdef _OPENMP
+    int prev = omp_get_max_threads();
+    if (n > 0) omp_set_num_threads(n);
+    return prev;
+#else
+    (void)n;
+    return 1;
+#endif
+}
+
+static double og_now(void) {
+#ifdef _OPENMP
+    return omp_get_wtime();
+#else
+    return 0.0;
+#endif
+}
 
 typedef struct {
     int imax, jmax, kmax, no_k;
@@ -103,6 +139,7 @@ static void neighbours(const og_dims *d, int i, int j, int k, int64_t nb[6]) {
 void oracle_matvec(const og_grid *g, const double *a, const double *x, double *y) {
     og_dims d = dims_of(g);
     const int64_t n3 = d.n3;
+#pragma omp parallel for collapse(2) schedule(static)
     for (int k = 1; k <= d.nk; ++k)
         for (int i = 1; i <= d.ni; ++i)
             for (int j = 1; j <= d.nj; ++j) {
@@ -260,6 +297,34 @@ static void rb_position(int p, int s1, int lo2, int *f, int *s) {
     *s = slow;
 }
 
+/* One red-black pass of J lines.  3-D (leqsol_mod.f:497-522): first = 1
+ * takes k odd -> i = 2, 4, ..; k even -> i = 1, 3, .. (1-based), first = 0
+ * the other colour.  2-D (:383-391): first = 1 takes i = 1, 3, .., first = 0
+ * i = 2, 4, ..  The lines of one pass read only lines of the other colour,
+ * so they are independent: the pass is a parallel loop, as in the
+ * reference, with private line buffers.  DGTSV failures leave the line
+ * unchanged (:910-914). */
+static void rb_pass(const og_dims *d, int three_d, int first, const double *a, const double *bm, double *var,
+                    int nmax) {
+#pragma omp parallel
+    {
+        double *buf = malloc(sizeof(double) * 4 * (size_t)nmax);
+        if (three_d) {
+#pragma omp for schedule(static)
+            for (int k = 1; k <= d->nk; ++k) {
+                const int i0 = ((k % 2 != 0) == (first != 0)) ? 2 : 1;
+                for (int i = i0; i <= d->ni; i += 2)
+                    sweep_j_line(d, i, k, a, bm, var, buf, buf + nmax, buf + 2 * nmax, buf + 3 * nmax);
+            }
+        } else {
+#pragma omp for schedule(static)
+            for (int i = first ? 1 : 2; i <= d->ni; i += 2)
+                sweep_j_line(d, i, 1, a, bm, var, buf, buf + nmax, buf + 2 * nmax, buf + 3 * nmax);
+        }
+        free(buf);
+    }
+}
+
 /* LEQ_MSOLVE leqsol_mod.f:286-598.  var = b (SETGUESS, :341-357), then one
  * pass per character of the 4-character CMETHOD.  Returns 0, or a positive
  * DGTSV INFO from a JK/IJ sweep, which the reference treats as fatal. */
@@ -280,8 +345,8 @@ int oracle_msolve_line(const og_grid *g, const double *bm, const double *a, doub
             if (do_i)
                 for (int i = 1; i <= d.ni; ++i) sweep_j_line(&d, i, 1, a, bm, var, cc, dd, ee, bb);
             if (do_rb) {
-                for (int i = 1; i <= d.ni; i += 2) sweep_j_line(&d, i, 1, a, bm, var, cc, dd, ee, bb);
-                for (int i = 2; i <= d.ni; i += 2) sweep_j_line(&d, i, 1, a, bm, var, cc, dd, ee, bb);
+                rb_pass(&d, 0, 1, a, bm, var, nmax);
+                rb_pass(&d, 0, 0, a, bm, var, nmax);
             }
             continue;
         }
@@ -317,10 +382,8 @@ int oracle_msolve_line(const og_grid *g, const double *bm, const double *a, doub
         }
         if (do_rb && !fatal) {
             /* :497-522: k odd -> i = istart+1, +3..; k even -> i = istart, +2.. */
-            for (int k = 1; k <= d.nk; ++k)
-                for (int i = (k % 2 != 0) ? 2 : 1; i <= d.ni; i += 2) sweep_j_line(&d, i, k, a, bm, var, cc, dd, ee, bb);
-            for (int k = 1; k <= d.nk; ++k)
-                for (int i = (k % 2 != 0) ? 1 : 2; i <= d.ni; i += 2) sweep_j_line(&d, i, k, a, bm, var, cc, dd, ee, bb);
+            rb_pass(&d, 1, 1, a, bm, var, nmax);
+            rb_pass(&d, 1, 0, a, bm, var, nmax);
         }
         if (!fatal) {
             /* USE_IKLOOP = .FALSE. branch, :570-585 */
@@ -341,6 +404,7 @@ int oracle_msolve_line(const og_grid *g, const double *bm, const double *a, doub
 void oracle_msolve_diag(const og_grid *g, const double *bm, const double *a, double *var) {
     og_dims d = dims_of(g);
     const int64_t n3 = d.n3;
+#pragma omp parallel for schedule(static)
     for (int64_t c = 0; c < n3; ++c) var[c] = bm[c] / A(0, c);
 }
 
@@ -393,7 +457,8 @@ void oracle_dot2(const og_grid *g, const double *r1, const double *r2, const dou
 int oracle_unit_scale(const og_grid *g, double *a, double *bm) {
     og_dims d = dims_of(g);
     const int64_t n3 = d.n3;
-    int ier = 0;
+    int zero = 0;
+#pragma omp parallel for schedule(static) reduction(| : zero)
     for (int64_t c = 0; c < n3; ++c) {
         double amax = 0.0;
         for (int l = -3; l <= 3; ++l) {
@@ -405,25 +470,30 @@ int oracle_unit_scale(const og_grid *g, double *a, double *bm) {
             for (int l = -3; l <= 3; ++l) A(l, c) = A(l, c) * oam;
             bm[c] = bm[c] * oam;
         } else {
-            ier = -2;
+            zero = 1;
         }
     }
-    return ier;
+    return zero ? -2 : 0;
 }
 
 typedef struct {
     int iterations, iter_tot_increment, ier, exit_reason;
     double rnorm0, rnorm;
     int msolve_fatal;
+    double seconds_setup;   /* wall time from the call to the end of the set-up (scaling, R, Rtilde) */
+    double seconds_total;   /* wall time of the whole call */
 } og_result;
 
 /* LEQ_BICGS (PC dispatch, leq_bicgs.f:74-87) + LEQ_BICGS0 (:107-509).
  * noise: the U[0,1) array random_number would fill Rtilde with (:298).
- * rnorm_trace (optional, itmax+1 entries) receives Rnorm0 then each Rnorm.
+ * rnorm_trace (optional, itmax+1 entries) receives Rnorm0 then each Rnorm;
+ * time_trace (optional, itmax+1 entries) the wall time since the call at the
+ * end of the set-up, then at the end of each iteration.
  * Returns 0, or -1 for an unknown PC (the reference aborts). */
 int oracle_leq_bicgs(const og_grid *g, const char *pc, const char *cmethod, double tol, int itmax,
                      double *var, double *a, double *bm, const double *noise, og_result *res,
-                     double *rnorm_trace) {
+                     double *rnorm_trace, double *time_trace) {
+    const double t_call = og_now();
     int mode;
     if (strncmp(pc, "LINE", 4) == 0) mode = 2;
     else if (strncmp(pc, "DIAG", 4) == 0) mode = 1;
@@ -434,7 +504,11 @@ int oracle_leq_bicgs(const og_grid *g, const char *pc, const char *cmethod, doub
     const int64_t n = d.n3;
     memset(res, 0, sizeof(*res));
     res->ier = oracle_unit_scale(g, a, bm);
-    if (res->ier != 0) { res->exit_reason = 4; return 0; }
+    if (res->ier != 0) {
+        res->exit_reason = 4;
+        res->seconds_total = og_now() - t_call;
+        return 0;
+    }
 
     double *R = calloc(n, 8), *Rt = calloc(n, 8), *P = calloc(n, 8), *Pp = calloc(n, 8);
     double *S = calloc(n, 8), *Sp = calloc(n, 8), *T = calloc(n, 8), *V = calloc(n, 8);
@@ -445,10 +519,14 @@ int oracle_leq_bicgs(const og_grid *g, const char *pc, const char *cmethod, doub
     int isconverged = 0, iter = 1, done = 0;
 
     oracle_matvec(g, a, var, R);
+#pragma omp parallel for schedule(static)
     for (int64_t c = 0; c < n; ++c) R[c] = bm[c] - R[c];
     rnorm0 = sqrt(oracle_dot(g, R, R));
+#pragma omp parallel for schedule(static)
     for (int64_t c = 0; c < n; ++c) Rt[c] = R[c] + (2.0 * noise[c] - 1.0) * 1.0e-6 * rnorm0;
     if (rnorm_trace) rnorm_trace[0] = rnorm0;
+    res->seconds_setup = og_now() - t_call;
+    if (time_trace) time_trace[0] = res->seconds_setup;
 
     double *Phat = P, *Shat = S;
     int i;
@@ -462,9 +540,11 @@ int oracle_leq_bicgs(const og_grid *g, const char *pc, const char *cmethod, doub
             break;
         }
         if (i == 1) {
-            memcpy(P, R, sizeof(double) * n);
+#pragma omp parallel for schedule(static)
+            for (int64_t c = 0; c < n; ++c) P[c] = R[c];
         } else {
             double beta = (rho[i - 1] / rho[i - 2]) * (alpha[i - 1] / omega[i - 1]);
+#pragma omp parallel for schedule(static)
             for (int64_t c = 0; c < n; ++c) P[c] = R[c] + beta * (P[c] - omega[i - 1] * V[c]);
         }
         if (use_pc) {
@@ -477,10 +557,12 @@ int oracle_leq_bicgs(const og_grid *g, const char *pc, const char *cmethod, doub
         oracle_matvec(g, a, Phat, V);
         double rtv = oracle_dot(g, Rt, V);
         alpha[i] = rho[i - 1] / rtv;
+#pragma omp parallel for schedule(static)
         for (int64_t c = 0; c < n; ++c) S[c] = R[c] - alpha[i] * V[c];
         if (!g->minimize_dotproducts) {
             snorm = sqrt(oracle_dot(g, S, S));
             if (snorm <= tolmin) {
+#pragma omp parallel for schedule(static)
                 for (int64_t c = 0; c < n; ++c) var[c] = var[c] + alpha[i] * Phat[c];
                 isconverged = 1;
                 res->exit_reason = 1;
@@ -508,7 +590,9 @@ int oracle_leq_bicgs(const og_grid *g, const char *pc, const char *cmethod, doub
         }
         if (txt == 0.0) txt = 1.0e-15; /* SMALL_NUMBER param1_mod.f:24 */
         omega[i] = txs / txt;
+#pragma omp parallel for schedule(static)
         for (int64_t c = 0; c < n; ++c) var[c] = var[c] + alpha[i] * Phat[c] + omega[i] * Shat[c];
+#pragma omp parallel for schedule(static)
         for (int64_t c = 0; c < n; ++c) R[c] = S[c] - omega[i] * T[c];
         int icheck = g->icheck_bicgs > 0 ? g->icheck_bicgs : 1;
         if (!g->minimize_dotproducts || (iter % icheck) == 0) {
@@ -519,9 +603,11 @@ int oracle_leq_bicgs(const og_grid *g, const char *pc, const char *cmethod, doub
                 res->iter_tot_increment = iter + 1;
                 res->exit_reason = 0;
                 res->iterations = i;
+                if (time_trace) time_trace[i] = og_now() - t_call;
                 break;
             }
         }
+        if (time_trace) time_trace[i] = og_now() - t_call;
         iter = iter + 1;
     }
     if (!done) {
@@ -536,6 +622,7 @@ int oracle_leq_bicgs(const og_grid *g, const char *pc, const char *cmethod, doub
     }
     res->rnorm0 = rnorm0;
     res->rnorm = rnorm;
+    res->seconds_total = og_now() - t_call;
     free(R); free(Rt); free(P); free(Pp); free(S); free(Sp); free(T); free(V);
     free(alpha); free(omega); free(rho);
     return 0;

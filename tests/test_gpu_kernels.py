@@ -126,3 +126,66 @@ def test_dot_product_par(gpu, oracle_mod, case, serial):
     bound = 1e-14 * np.sum(np.abs(x * y))
     assert abs(got - exact) <= bound
     assert abs(ref - exact) <= bound
+
+
+# J lines longer than one 16-row tile of the red-black kernel (nj >= 33:
+# several tiles, and a partial last tile), in 3-D, 2-D and with a cyclic axis
+TILED = [
+    ("m_11x45x9", 1, 11, 45, 9, False, (0, 0, 0)),
+    ("p_14x34x7_cycz", 0, 14, 34, 7, False, (0, 0, 1)),
+    ("m_9x127x6", 1, 9, 127, 6, False, (0, 0, 0)),
+    ("m_20x50_2d", 1, 20, 50, 1, True, (0, 0, 0)),
+]
+TILED_CASES = [(c, m) for c in TILED for m in ("RSRS", "RSSS")]
+
+
+@pytest.mark.parametrize("case,cmethod", TILED_CASES, ids=[f"{c[0]}-{m}" for c, m in TILED_CASES])
+def test_line_msolve_multi_tile_bit_exact(gpu, oracle_mod, case, cmethod):
+    """R/S sweep strings run the solver's red-black kernel (the one every
+    LINE solve uses), whose lines here span several 16-row tiles."""
+    name, kind, im, jm, km, no_k, cyc = case
+    print(f"multi-tile line solve {name} {cmethod}", flush=True)
+    og, a, b = _setup(gpu, oracle_mod, kind, im, jm, km, no_k, cyc)
+    a, b, _ = oracle_mod.unit_scale(og, a, b)
+    r = rng_vec(b.size, 13)
+    z_ref, info = oracle_mod.msolve(og, "LINE", a, r, cmethod)
+    assert info == 0
+    da, dr = gpu.DeviceArray.from_host(a), gpu.DeviceArray.from_host(r)
+    dz = gpu.DeviceArray(r.size)
+    gpu.msolve_dev("LINE", cmethod, da, dr, dz)
+    np.testing.assert_array_equal(dz.download(), z_ref)
+
+
+def test_line_msolve_multi_tile_pivoting(gpu, oracle_mod):
+    """Row interchanges on lines of several tiles, partial last tile."""
+    og, a, b = _setup(gpu, oracle_mod, 1, 8, 40, 6, False, (0, 0, 0))
+    n3 = b.size
+    a2 = a.reshape(7, n3)
+    sel = np.random.default_rng(3).random(n3) < 0.3
+    a2[1, sel] *= 25.0
+    a2[5, sel] *= 25.0
+    r = rng_vec(n3, 4)
+    z_ref, info = oracle_mod.msolve(og, "LINE", a, r, "RSRS")
+    da, dr = gpu.DeviceArray.from_host(a), gpu.DeviceArray.from_host(r)
+    dz = gpu.DeviceArray(n3)
+    gpu.msolve_dev("LINE", "RSRS", da, dr, dz)
+    np.testing.assert_array_equal(dz.download(), z_ref)
+
+
+@pytest.mark.parametrize("serial", [True, False])
+def test_dot_product_multi_block_finish(gpu, oracle_mod, serial):
+    """66^3 (287,496 cells): more than 1024 partials per reduction, so the
+    finish runs in several blocks (level-2 sums, release increment, acquire
+    in the last block)."""
+    og, a, b = _setup(gpu, oracle_mod, 0, 66, 66, 66, False, (0, 0, 0), is_serial=serial)
+    x, y = rng_vec(b.size, 21), rng_vec(b.size, 22)
+    dx, dy = gpu.DeviceArray.from_host(x), gpu.DeviceArray.from_host(y)
+    for rep in range(3):  # the finish counter is reset by each launch's last block
+        got = gpu.dot_dev(dx, dy)
+        if serial:
+            exact = math.fsum(x * y)
+        else:
+            m = np.zeros((68, 68, 68), bool)
+            m[1:-1, 1:-1, 1:-1] = True
+            exact = math.fsum((x * y)[m.reshape(-1)])
+        assert abs(got - exact) <= 1e-14 * np.sum(np.abs(x * y)), (rep, got, exact)
