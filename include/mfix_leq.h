@@ -147,6 +147,17 @@ int mfix_leq_scale_dev(double *d_a_m, double *d_b_m, int *ier, void *stream);
  * x_c, products in FP32, sums in FP64, one rounding to FP32. */
 int mfix_leq_matvec32_dev(const float *d_a_m, const float *d_x, float *d_y,
                           void *stream);
+/* LINE preconditioner of the mixed-precision inner solver, the FP32
+ * counterpart of LEQ_MSOLVE (model/leqsol_mod.f:286-598): z = r, then the
+ * red-black J-line passes of `cmethod` (R/S characters; others run one line
+ * per thread).  d_a_m holds 8 FP32 planes of the current grid: the six
+ * couplings at planes 0-2, 4-6 as in A_M and A0 at plane 7 (plane 3, the
+ * row excess, is not read).  Each line of a red-black pass is split over
+ * several lanes (partitioned elimination + cyclic reduction, no pivoting),
+ * so the result agrees with the FP64 line solves to FP32 rounding, not bit
+ * for bit. */
+int mfix_leq_msolve32_dev(const char *cmethod, const float *d_a_m, const float *d_r, float *d_z,
+                          void *stream);
 
 /* ---- synthetic systems (host only, deterministic, no GPU needed) -------- */
 /* kind 0: symmetric pressure-correction, faces 10^U[-1,1], A0=-sum-1e-3U, B=U[-1,1]

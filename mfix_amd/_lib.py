@@ -168,6 +168,7 @@ _SIGS = {
     "mfix_leq_prof_enable": (C.c_int, [C.c_int, C.c_int]),
     "mfix_leq_prof_read": (C.c_int, [C.c_int, C.POINTER(C.c_int64), c_double_p, c_double_p]),
     "mfix_leq_prof_idle": (C.c_int, [C.c_int, C.POINTER(C.c_int64), c_double_p]),
+    "mfix_leq_msolve32_dev": (C.c_int, [C.c_char_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
 }
 
 EXPORTED = tuple(_SIGS)

@@ -214,6 +214,33 @@ int mfix_leq_matvec_dev(const double *d_a_m, const double *d_x, double *d_y, voi
     return sync_status(s, "matvec");
 }
 
+int mfix_leq_msolve32_dev(const char *cmethod, const float *d_a_m, const float *d_r, float *d_z, void *stream) {
+    hipStream_t s;
+    int rc;
+    if ((rc = ready(&s, stream))) return rc;
+    if (!d_a_m || !d_r || !d_z) return MFIX_LEQ_EINVAL;
+    Context &C = ctx();
+    const DGrid g = make_dgrid(C.grid);
+    if ((rc = C.tmp.ensure(sizeof(float) * (size_t)line_scratch_elems(g)))) return rc;
+    if ((rc = C.misc.ensure(sizeof(int) * 4 + sizeof(double) * 8))) return rc;
+    const char *cm = cmethod ? cmethod : "RSRS";
+    rc = run_msolve_line_rb<float>(g, cm, d_a_m, d_r, d_z, (float *)C.tmp.ptr, s);
+    if (rc < 0) return rc;
+    if (rc == 1) {
+        int *flag = (int *)C.misc.ptr;
+        hipError_t e = hipMemsetAsync(flag, 0, sizeof(int), s);
+        if (e != hipSuccess) return hip_fail(e, "memset");
+        launch_msolve_line<float>(g, C.shape_for(g.n3, true), cm, d_a_m, d_r, d_z, (float *)C.tmp.ptr, nullptr, 0,
+                                  flag, s, true);
+        int hflag = 0;
+        e = hipMemcpyAsync(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost, s);
+        if (e != hipSuccess) return hip_fail(e, "copy");
+        if ((rc = sync_status(s, "msolve32"))) return rc;
+        return hflag ? MFIX_LEQ_EDGTSV : 0;
+    }
+    return sync_status(s, "msolve32");
+}
+
 int mfix_leq_matvec32_dev(const float *d_a_m, const float *d_x, float *d_y, void *stream) {
     hipStream_t s;
     int rc;
@@ -250,7 +277,7 @@ int mfix_leq_msolve_dev(const char *pc, const char *cmethod, const double *d_a_m
     if (e != hipSuccess) return hip_fail(e, "memset");
     // R/S sweep strings: the solver's own red-black kernel; others the
     // per-line phases
-    rc = run_msolve_line_rb(g, cmethod ? cmethod : "RSRS", d_a_m, d_r, d_z, (double *)C.tmp.ptr, s);
+    rc = run_msolve_line_rb<double>(g, cmethod ? cmethod : "RSRS", d_a_m, d_r, d_z, (double *)C.tmp.ptr, s);
     if (rc < 0) return rc;
     if (rc == 1)
         launch_msolve_line<double>(g, C.shape_for(g.n3, false), cmethod ? cmethod : "RSRS", d_a_m, d_r, d_z,

@@ -64,6 +64,9 @@ using Sweep = std::array<char, 4>;
 struct MapSet {
     std::vector<int> ids;
     std::vector<int2> smap, fmap, umap, lmap[kMaxRPasses];
+    // FP32 tables: red-black passes of the several-lanes-per-line kernel,
+    // one map per kernel variant (line length class) and pass
+    std::vector<int2> pmap[kPcrVariants][kMaxRPasses];
     int nlpass = 0;
     std::vector<int> line_seq;
     double cells = 0.0;           // sum of N3 over ids (timing bytes)
@@ -72,6 +75,7 @@ struct MapSet {
     DevBuf dev;
     int *d_ids = nullptr;
     int2 *d_smap = nullptr, *d_fmap = nullptr, *d_umap = nullptr, *d_lmap[kMaxRPasses] = {};
+    int2 *d_pmap[kPcrVariants][kMaxRPasses] = {};
     uint64_t version = 0;         // bumped whenever the device copy changes
 };
 
@@ -153,16 +157,23 @@ size_t sys_bytes(const DGrid &g, const Part &P, int pc, bool mixed);
 template <typename T>
 void sys_carve(Sys<T> &S, const SysDesc &d, const Part &P, char *base);
 
+// With ext (a device region of ext_cap bytes), build_maps and upload_table
+// write the block maps / the table there instead of into their own
+// grow-only buffers: nothing is allocated.
 template <typename T>
-int build_maps(const Group<T> &G, const std::vector<int> &ids, MapSet &M, hipStream_t s);
+int build_maps(const Group<T> &G, const std::vector<int> &ids, MapSet &M, hipStream_t s, char *ext = nullptr,
+               size_t ext_cap = 0);
 template <typename T>
-int upload_table(Group<T> &G, hipStream_t s);
+int upload_table(Group<T> &G, hipStream_t s, char *ext = nullptr, size_t ext_cap = 0);
 template <typename T>
 void enqueue_setup(Group<T> &G, const MapSet &M, hipStream_t s);
 template <typename T>
 void enqueue_iterations(Group<T> &G, const MapSet &M, int n, hipStream_t s);
+// d_count (kCountBytes of device memory; NULL: the context's own buffer)
+// receives the running-system count and flags polled between chunks.
+constexpr size_t kCountBytes = 256 + 2 * sizeof(int) * kHostStates;
 template <typename T>
-int run_group(Group<T> &G, int max_iters, hipStream_t s, bool graphs);
+int run_group(Group<T> &G, int max_iters, hipStream_t s, bool graphs, int *d_count = nullptr);
 template <typename T>
 int read_states(const Group<T> &G, const std::vector<int> &ids, SolverState *h, hipStream_t s);
 
@@ -185,9 +196,11 @@ int run_dot(const DGrid &g, const double *x, const double *y, double *d_out, hip
 int run_scale(const DGrid &g, double *a, double *b, int *zero_row, hipStream_t s);
 int run_copy(const double *src, double *dst, int64_t n, hipStream_t s);
 // LEQ_MSOLVE (LINE) with a sweep string of R/S passes through the batched
-// red-black kernel; returns 1 (nothing launched) for other sweep strings.
-int run_msolve_line_rb(const DGrid &g, const char *cmethod, const double *a, const double *r, double *z,
-                       double *scratch, hipStream_t s);
+// red-black kernels (FP64: DGTSV per line; FP32 difference layout: several
+// lanes per line); returns 1 (nothing launched) for other sweep strings.
+template <typename T>
+int run_msolve_line_rb(const DGrid &g, const char *cmethod, const T *a, const T *r, T *z, T *scratch,
+                       hipStream_t s);
 int autotune(int imax, int jmax, int kmax, int mixed, Shape *best);
 // CALC_RESID_PP of device arrays on the current grid (leq_resid.hip).
 int resid_pp(const double *b, const int *flag, double norm, int debug, mfix_leq_resid *out, hipStream_t s);

@@ -848,6 +848,207 @@ __global__ void __launch_bounds__(192) k_line_color_sys(const Sys<T> *__restrict
     }
 }
 
+// Red-black line phase along J for the FP32 inner solve of the mixed path:
+// several lanes per line.  The FP64 kernel above must repeat DGTSV's
+// elimination order to reproduce the reference bit for bit, one serial chain
+// per line; the FP32 recurrence is corrected by the FP64 reliable updates, so
+// here each line of n rows is split over P lanes of m = ceil(n/P) <= M
+// consecutive rows (partition method, H. H. Wang, ACM TOMS 7 (1981) 170-183):
+//   1. each lane eliminates its chunk's rows but the last (the separator),
+//      writing every one as x_j = U_j - V_j G_l - W_j G_{l-1} in terms of
+//      its own separator G_l and the previous lane's G_{l-1};
+//   2. substituting into the separator rows leaves a tridiagonal system in
+//      the P separators, solved by parallel cyclic reduction across the lanes
+//      (log2 P steps of lane shuffles; Hockney 1965);
+//   3. each lane recovers its rows from G_l and G_{l-1}.
+// No pivoting: MFIX's rows are diagonally dominant, and a line whose result
+// is not finite keeps its values, as a failed DGTSV leaves it
+// (leqsol_mod.f:910-914).  The loads are the line's own rows, contiguous in
+// J, so a lane group reads whole line segments and nothing is staged through
+// LDS or scratch.  Rows past n are identity rows (x = 0, no coupling: the
+// last real row has no upper coupling).  Block b takes lines
+// [map[b].y*LPB, +LPB) of colour `param` (numbered densely by color_line), four
+// waves of 64/P lines.  which = 0 preconditions P into Phat, 1 S into Shat.
+template <int P, int M>
+__global__ void __launch_bounds__(256) k_line_pcr_sys(const Sys<float> *__restrict__ sys,
+                                                      const int2 *__restrict__ map, int param, int which) {
+    constexpr int LPW = 64 / P;
+    constexpr int LPB = 4 * LPW;
+    const int2 e = map[blockIdx.x];
+    if (e.x < 0) return;
+    const Sys<float> &S = sys[e.x];
+    auto *__restrict__ st = to_global(S.st);
+    if (st->done || (which == 1 && st->s_exit)) return;
+    if (e.y == 0 && threadIdx.x == 0) st->line_launches += 1;  // (timing: working launches)
+    const DGrid &g = S.g;
+    const int n = g.nj;
+    const int m = (n + P - 1) / P;  // rows per lane, <= M (the host picks the variant)
+    const int64_t n3 = g.n3;
+    const int lane = threadIdx.x & 63;
+    const int grp = lane / P, gl = lane - grp * P, base = grp * P;
+    const int64_t t = (int64_t)e.y * LPB + (threadIdx.x >> 6) * LPW + grp;
+    int u = 0, v = 0;
+    const bool ok = color_line(g.ni, g.nk, g.do_k ? 1 : 0, param, t, u, v);
+    if (!ok) u = v = 0;  // a group past the colour's last line addresses line (0, 0) and stores nothing
+    const uint32_t c1 = (uint32_t)g.c1, c2 = (uint32_t)g.c2;
+    const uint32_t c0 = (uint32_t)u * c1 + (uint32_t)v * c2;
+    const uint32_t oim = (uint32_t)(nb_minus(u, g.ni, g.cyc_x) - u) * c1;
+    const uint32_t oip = (uint32_t)(nb_plus(u, g.ni, g.cyc_x) - u) * c1;
+    const uint32_t okm = g.do_k ? (uint32_t)(nb_minus(v, g.nk, g.cyc_z) - v) * c2 : 0u;
+    const uint32_t okp = g.do_k ? (uint32_t)(nb_plus(v, g.nk, g.cyc_z) - v) * c2 : 0u;
+    const gptr<const float> a = to_global(S.a);
+    const gptr<const float> pK_m = a, pJ_m = a + n3, pI_m = a + 2 * n3, pI_p = a + 4 * n3, pJ_p = a + 5 * n3;
+    const gptr<const float> pK_p = a + 6 * n3, pD = a + (int64_t)DiagPlane<float>::value * n3;
+    const gptr<const float> bm = to_global((const float *)(which == 0 ? S.p : S.s));
+    const gptr<float> var = to_global(which == 0 ? S.phat : S.shat);
+    const int do_k = g.do_k;
+    // rows q = gl*m + j of this lane: loads issued together, then the
+    // right-hand side in LEQ_IKSWEEP's order (leqsol_mod.f:891-902)
+    float ra[M], rb[M], rc[M], rd[M];
+    {
+        float tb[M], t1m[M], t1p[M], v1m[M], v1p[M], t2m[M], t2p[M], v2m[M], v2p[M];
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+            const int q = gl * m + j;
+            const bool in = ok && j < m && q < n;
+            const uint32_t c = in ? c0 + (uint32_t)q : c0;  // in bounds even when not used
+            tb[j] = in ? bm[c] : 0.f;
+            t1m[j] = in ? pI_m[c] : 0.f;
+            t1p[j] = in ? pI_p[c] : 0.f;
+            v1m[j] = in ? var[c + oim] : 0.f;
+            v1p[j] = in ? var[c + oip] : 0.f;
+            t2m[j] = in && do_k ? pK_m[c] : 0.f;
+            t2p[j] = in && do_k ? pK_p[c] : 0.f;
+            v2m[j] = in && do_k ? var[c + okm] : 0.f;
+            v2p[j] = in && do_k ? var[c + okp] : 0.f;
+            rb[j] = in ? pD[c] : 1.f;
+            ra[j] = in && q > 0 ? pJ_m[c] : 0.f;
+            rc[j] = in && q < n - 1 ? pJ_p[c] : 0.f;
+        }
+#pragma unroll
+        for (int j = 0; j < M; ++j)
+            rd[j] = do_k ? tb[j] - t1m[j] * v1m[j] - t1p[j] * v1p[j] - t2m[j] * v2m[j] - t2p[j] * v2p[j]
+                         : tb[j] - t1m[j] * v1m[j] - t1p[j] * v1p[j];
+    }
+    // 1. rows 0 .. m-2 of the chunk: forward elimination carrying the
+    // coupling to the previous lane's separator, then backward substitution
+    float cp[M], wp[M], dp[M], U[M], V[M], W[M];
+#pragma unroll
+    for (int j = 0; j < M - 1; ++j) {
+        if (j < m - 1) {
+            if (j == 0) {
+                const float r = 1.f / rb[0];
+                cp[0] = rc[0] * r;
+                wp[0] = ra[0] * r;
+                dp[0] = rd[0] * r;
+            } else {
+                const float r = 1.f / (rb[j] - ra[j] * cp[j - 1]);
+                cp[j] = rc[j] * r;
+                wp[j] = -ra[j] * wp[j - 1] * r;
+                dp[j] = (rd[j] - ra[j] * dp[j - 1]) * r;
+            }
+        }
+    }
+    float Ul = 0.f, Vl = 0.f, Wl = 0.f;  // row m-2 (the one next to the separator)
+#pragma unroll
+    for (int j = M - 2; j >= 0; --j) {
+        if (j < m - 1) {
+            if (j == m - 2) {
+                U[j] = dp[j];
+                V[j] = cp[j];
+                W[j] = wp[j];
+                Ul = U[j];
+                Vl = V[j];
+                Wl = W[j];
+            } else {
+                U[j] = dp[j] - cp[j] * U[j + 1];
+                V[j] = -cp[j] * V[j + 1];
+                W[j] = wp[j] - cp[j] * W[j + 1];
+            }
+        }
+    }
+    // the separator row (row m-1 of the chunk)
+    float as = 0.f, bs = 1.f, cs = 0.f, ds = 0.f;
+#pragma unroll
+    for (int j = 0; j < M; ++j)
+        if (j == m - 1) {
+            as = ra[j];
+            bs = rb[j];
+            cs = rc[j];
+            ds = rd[j];
+        }
+    // 2. reduced system A G_{l-1} + B G_l + C G_{l+1} = D over the P lanes
+    const int nxt = base + (gl + 1 < P ? gl + 1 : gl);
+    float U0 = m > 1 ? U[0] : 0.f, V0 = m > 1 ? V[0] : 0.f, W0 = m > 1 ? W[0] : 0.f;
+    float Un = __shfl(U0, nxt, 64), Vn = __shfl(V0, nxt, 64), Wn = __shfl(W0, nxt, 64);
+    if (gl + 1 >= P) Un = Vn = Wn = 0.f;
+    float A, B, Cc, D;
+    if (m > 1) {
+        A = -as * Wl;
+        B = bs - as * Vl - cs * Wn;
+        Cc = -cs * Vn;
+        D = ds - as * Ul - cs * Un;
+    } else {
+        A = as;
+        B = bs;
+        Cc = cs;
+        D = ds;
+    }
+#pragma unroll
+    for (int sft = 1; sft < P; sft <<= 1) {
+        const bool hl = gl >= sft, hr = gl + sft < P;
+        const int sl = base + (hl ? gl - sft : gl), sr = base + (hr ? gl + sft : gl);
+        const float Al = __shfl(A, sl, 64), Bl = __shfl(B, sl, 64), Cl = __shfl(Cc, sl, 64), Dl = __shfl(D, sl, 64);
+        const float Ar = __shfl(A, sr, 64), Br = __shfl(B, sr, 64), Cr = __shfl(Cc, sr, 64), Dr = __shfl(D, sr, 64);
+        const float al = hl ? -A / Bl : 0.f;
+        const float ga = hr ? -Cc / Br : 0.f;
+        const float nA = hl ? al * Al : 0.f;
+        const float nC = hr ? ga * Cr : 0.f;
+        const float nB = B + (hl ? al * Cl : 0.f) + (hr ? ga * Ar : 0.f);
+        const float nD = D + (hl ? al * Dl : 0.f) + (hr ? ga * Dr : 0.f);
+        A = nA;
+        B = nB;
+        Cc = nC;
+        D = nD;
+    }
+    const float G = D / B;
+    float Gp = __shfl(G, base + (gl > 0 ? gl - 1 : 0), 64);
+    if (gl == 0) Gp = 0.f;
+    // 3. the chunk's rows; a line with a non-finite value keeps its values
+    float x[M];
+    int bad = 0;
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+        x[j] = j < m - 1 ? U[j] - V[j] * G - W[j] * Gp : G;
+        const int q = gl * m + j;
+        if (ok && j < m && q < n) bad |= !isfinite(x[j]);
+    }
+    const uint64_t bm_bad = __ballot(bad);
+    const uint64_t gmask = P == 64 ? ~0ull : (((1ull << P) - 1ull) << base);
+    if (!ok || (bm_bad & gmask)) return;
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+        const int q = gl * m + j;
+        if (j < m && q < n) var[c0 + (uint32_t)q] = x[j];
+    }
+}
+
+// Variant of k_line_pcr_sys for a line of n rows: lanes per line P and most
+// rows per lane M; -1 when n is too long for the registers of one lane.
+__host__ __device__ inline int line_pcr_variant(int n) {
+    if (n <= 32) return 0;   // P 16, M 2
+    if (n <= 64) return 1;   // P 16, M 4
+    if (n <= 128) return 2;  // P 16, M 8
+    if (n <= 256) return 3;  // P 32, M 8
+    if (n <= 320) return 4;  // P 64, M 5
+    if (n <= 512) return 5;  // P 64, M 8
+    return -1;
+}
+constexpr int kPcrVariants = 6;
+__host__ __device__ inline int line_pcr_lanes(int var) { return var <= 2 ? 16 : var == 3 ? 32 : 64; }
+// lines per block of a variant (four waves of 64/P lines)
+__host__ __device__ inline int line_pcr_lpb(int var) { return 4 * (64 / line_pcr_lanes(var)); }
+
 // Number of the listed systems still running, and a running flag per
 // listed system.
 template <typename T>

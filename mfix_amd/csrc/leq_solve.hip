@@ -297,13 +297,17 @@ int launch_msolve_line(const DGrid &g, const Shape &sh, const char *cmethod, con
 
 // LEQ_MSOLVE (leqsol_mod.f:286-598) with a sweep string whose characters are
 // R passes or characters that do nothing here (S; J; K and A in 2-D),
-// through the red-black kernel the solver uses (k_line_color_sys): z = r
-// (SETGUESS, :341-357), then per R pass the (i+k)-odd colour and the even
-// one.  A one-system table carries the arrays; its state is blank, so the
-// kernel never returns early.  Returns 1, launching nothing, for a sweep
-// string with I/K/A passes (the caller runs k_line_phase instead).
-int run_msolve_line_rb(const DGrid &g, const char *cmethod, const double *a, const double *r, double *z,
-                       double *scratch, hipStream_t s) {
+// through the red-black kernels the solver uses: z = r (SETGUESS,
+// :341-357), then per R pass the (i+k)-odd colour and the even one.  FP64:
+// k_line_color_sys (DGTSV per line, the reference's bits); FP32 (A in the
+// inner solve's difference layout, A0 in plane 7): k_line_pcr_sys where the
+// line fits one of its variants.  A one-system table carries the arrays; its
+// state is blank, so the kernels never return early.  Returns 1, launching
+// nothing, for a sweep string with I/K/A passes (the caller runs
+// k_line_phase instead).
+template <typename T>
+int run_msolve_line_rb(const DGrid &g, const char *cmethod, const T *a, const T *r, T *z, T *scratch,
+                       hipStream_t s) {
     Sweep sw = {'R', 'S', 'R', 'S'};
     if (cmethod)
         for (int q = 0; q < 4 && cmethod[q]; ++q) sw[q] = cmethod[q];
@@ -316,17 +320,19 @@ int run_msolve_line_rb(const DGrid &g, const char *cmethod, const double *a, con
     Context &C = ctx();
     const LineGeom L = line_geom(g, AX_J);
     const int64_t lines = ((int64_t)L.nu * L.nv + 1) / 2;
-    const int64_t lb = (lines + 63) / 64;
-    const size_t o_st = align256(sizeof(Sys<double>));
+    const int var = sizeof(T) == sizeof(float) ? line_pcr_variant(L.n) : -1;
+    const int lpb = var >= 0 ? line_pcr_lpb(var) : 64;
+    const int64_t lb = (lines + lpb - 1) / lpb;
+    const size_t o_st = align256(sizeof(Sys<T>));
     const size_t o_map = o_st + align256(sizeof(SolverState));
     int rc;
     if ((rc = C.line1.ensure(o_map + sizeof(int2) * (size_t)lb))) return rc;
     char *base = (char *)C.line1.ptr;
-    Sys<double> S = {};
+    Sys<T> S = {};
     S.g = g;
     S.pc = PC_LINE;
     S.a = a;
-    S.p = const_cast<double *>(r);
+    S.p = const_cast<T *>(r);
     S.phat = z;
     S.line_scratch = scratch;
     S.st = (SolverState *)(base + o_st);
@@ -334,21 +340,40 @@ int run_msolve_line_rb(const DGrid &g, const char *cmethod, const double *a, con
     for (int64_t b = 0; b < lb; ++b) map[(size_t)b] = make_int2(0, (int)b);
     hipError_t e = hipMemcpyAsync(base, &S, sizeof(S), hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(base + o_map, map.data(), sizeof(int2) * map.size(), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(z, r, sizeof(T) * (size_t)g.n3, hipMemcpyDeviceToDevice, s);
     if (e != hipSuccess) return hip_fail(e, "line table upload");
     hipLaunchKernelGGL(k_init_state, dim3(1), dim3(1), 0, s, S.st);
-    run_copy(r, z, g.n3, s);
     Prof &pf = C.prof[1];
+    const Sys<T> *tab = (const Sys<T> *)base;
+    const int2 *mp = (const int2 *)(base + o_map);
+    const dim3 gs((unsigned)lb);
     for (int q = 0; q < np; ++q)
         for (int param : {1, 0}) {
             pf.begin(s);
-            hipLaunchKernelGGL(k_line_color_sys<double>, dim3((unsigned)lb), dim3(192), 0, s,
-                               (const Sys<double> *)base, (const int2 *)(base + o_map), param, 0);
-            pf.end(s, 0.5 * (double)g.n3 * 13.0 * sizeof(double));
+            if constexpr (sizeof(T) == sizeof(float)) {
+                const Sys<float> *tf = (const Sys<float> *)tab;
+                switch (var) {
+                case 0: hipLaunchKernelGGL((k_line_pcr_sys<16, 2>), gs, dim3(256), 0, s, tf, mp, param, 0); break;
+                case 1: hipLaunchKernelGGL((k_line_pcr_sys<16, 4>), gs, dim3(256), 0, s, tf, mp, param, 0); break;
+                case 2: hipLaunchKernelGGL((k_line_pcr_sys<16, 8>), gs, dim3(256), 0, s, tf, mp, param, 0); break;
+                case 3: hipLaunchKernelGGL((k_line_pcr_sys<32, 8>), gs, dim3(256), 0, s, tf, mp, param, 0); break;
+                case 4: hipLaunchKernelGGL((k_line_pcr_sys<64, 5>), gs, dim3(256), 0, s, tf, mp, param, 0); break;
+                case 5: hipLaunchKernelGGL((k_line_pcr_sys<64, 8>), gs, dim3(256), 0, s, tf, mp, param, 0); break;
+                default: hipLaunchKernelGGL(k_line_color_sys<float>, gs, dim3(192), 0, s, tf, mp, param, 0); break;
+                }
+            } else {
+                hipLaunchKernelGGL(k_line_color_sys<T>, gs, dim3(192), 0, s, tab, mp, param, 0);
+            }
+            pf.end(s, 0.5 * (double)g.n3 * 13.0 * sizeof(T));
         }
     // the table must outlive the launches before it is rewritten
     e = hipStreamSynchronize(s);
     return e == hipSuccess ? 0 : hip_fail(e, "line solve");
 }
+template int run_msolve_line_rb<double>(const DGrid &, const char *, const double *, const double *, double *,
+                                        double *, hipStream_t);
+template int run_msolve_line_rb<float>(const DGrid &, const char *, const float *, const float *, float *, float *,
+                                       hipStream_t);
 
 template int launch_msolve_line<double>(const DGrid &, const Shape &, const char *, const double *, const double *,
                                         double *, double *, const SolverState *, int, int *, hipStream_t, bool);
@@ -461,8 +486,8 @@ static int rb_passes(const Sweep &sw, bool do_k) {
 }
 
 template <typename T>
-int build_maps(const Group<T> &G, const std::vector<int> &ids, MapSet &M, hipStream_t s) {
-    std::vector<int2> smap, fmap, umap, lmap[kMaxRPasses];
+int build_maps(const Group<T> &G, const std::vector<int> &ids, MapSet &M, hipStream_t s, char *ext, size_t ext_cap) {
+    std::vector<int2> smap, fmap, umap, lmap[kMaxRPasses], pmap[kPcrVariants][kMaxRPasses];
     std::vector<int> seq;
     int nlpass = 0, n_line = 0, n_fused = 0;
     double cells = 0.0, k1 = 0.0;
@@ -497,9 +522,14 @@ int build_maps(const Group<T> &G, const std::vector<int> &ids, MapSet &M, hipStr
             } else {
                 const LineGeom L = line_geom(S.g, AX_J);
                 const int64_t lines = ((int64_t)L.nu * L.nv + 1) / 2;
-                const int64_t lb = (lines + 63) / 64;
+                // FP32 (the mixed path's inner solve): several lanes per line
+                // where the line fits a variant; FP64: DGTSV lane per line
+                const int var = sizeof(T) == sizeof(float) ? line_pcr_variant(L.n) : -1;
+                const int lpb = var >= 0 ? line_pcr_lpb(var) : 64;
+                const int64_t lb = (lines + lpb - 1) / lpb;
                 for (int q = 0; q < np; ++q)
-                    for (int64_t b = 0; b < lb; ++b) lmap[q].push_back(make_int2(id, (int)b));
+                    for (int64_t b = 0; b < lb; ++b)
+                        (var >= 0 ? pmap[var][q] : lmap[q]).push_back(make_int2(id, (int)b));
                 nlpass = std::max(nlpass, np);
             }
         }
@@ -510,19 +540,26 @@ int build_maps(const Group<T> &G, const std::vector<int> &ids, MapSet &M, hipStr
                      !std::memcmp(M.fmap.data(), fmap.data(), fmap.size() * sizeof(int2)) &&
                      M.umap.size() == umap.size() &&
                      !std::memcmp(M.umap.data(), umap.data(), umap.size() * sizeof(int2));
-    for (int q = 0; q < kMaxRPasses && same; ++q)
+    for (int q = 0; q < kMaxRPasses && same; ++q) {
         same = M.lmap[q].size() == lmap[q].size() &&
                !std::memcmp(M.lmap[q].data(), lmap[q].data(), lmap[q].size() * sizeof(int2));
+        for (int v = 0; v < kPcrVariants && same; ++v)
+            same = M.pmap[v][q].size() == pmap[v][q].size() &&
+                   !std::memcmp(M.pmap[v][q].data(), pmap[v][q].data(), pmap[v][q].size() * sizeof(int2));
+    }
     M.cells = cells;
     M.k1_bytes = k1;
     M.n_line = n_line;
     M.n_fused = n_fused;
-    if (same) return 0;
+    if (same && !ext) return 0;
     M.ids = ids;
     M.smap.swap(smap);
     M.fmap.swap(fmap);
     M.umap.swap(umap);
-    for (int q = 0; q < kMaxRPasses; ++q) M.lmap[q].swap(lmap[q]);
+    for (int q = 0; q < kMaxRPasses; ++q) {
+        M.lmap[q].swap(lmap[q]);
+        for (int v = 0; v < kPcrVariants; ++v) M.pmap[v][q].swap(pmap[v][q]);
+    }
     M.nlpass = nlpass;
     M.line_seq.swap(seq);
     // one device block: ids | smap | fmap | umap | lmap[0..3]
@@ -537,39 +574,55 @@ int build_maps(const Group<T> &G, const std::vector<int> &ids, MapSet &M, hipStr
     const size_t o_s = put(M.smap.data(), M.smap.size() * sizeof(int2));
     const size_t o_f = put(M.fmap.data(), M.fmap.size() * sizeof(int2));
     const size_t o_u = put(M.umap.data(), M.umap.size() * sizeof(int2));
-    size_t o_l[kMaxRPasses];
+    size_t o_l[kMaxRPasses], o_p[kPcrVariants][kMaxRPasses];
     for (int q = 0; q < kMaxRPasses; ++q) o_l[q] = put(M.lmap[q].data(), M.lmap[q].size() * sizeof(int2));
+    for (int v = 0; v < kPcrVariants; ++v)
+        for (int q = 0; q < kMaxRPasses; ++q) o_p[v][q] = put(M.pmap[v][q].data(), M.pmap[v][q].size() * sizeof(int2));
     int rc;
-    if ((rc = M.dev.ensure(std::max<size_t>(blob.size(), 256)))) return rc;
-    hipError_t e = hipMemcpyAsync(M.dev.ptr, blob.data(), blob.size(), hipMemcpyHostToDevice, s);
+    char *base = ext;
+    if (ext) {
+        if (blob.size() > ext_cap) return MFIX_LEQ_ENOMEM;
+    } else {
+        if ((rc = M.dev.ensure(std::max<size_t>(blob.size(), 256)))) return rc;
+        base = (char *)M.dev.ptr;
+    }
+    hipError_t e = hipMemcpyAsync(base, blob.data(), blob.size(), hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_fail(e, "block map upload");
-    char *base = (char *)M.dev.ptr;
     M.d_ids = (int *)(base + o_ids);
     M.d_smap = (int2 *)(base + o_s);
     M.d_fmap = (int2 *)(base + o_f);
     M.d_umap = (int2 *)(base + o_u);
     for (int q = 0; q < kMaxRPasses; ++q) M.d_lmap[q] = (int2 *)(base + o_l[q]);
+    for (int v = 0; v < kPcrVariants; ++v)
+        for (int q = 0; q < kMaxRPasses; ++q) M.d_pmap[v][q] = (int2 *)(base + o_p[v][q]);
     M.version += 1;
     return 0;
 }
-template int build_maps<double>(const Group<double> &, const std::vector<int> &, MapSet &, hipStream_t);
-template int build_maps<float>(const Group<float> &, const std::vector<int> &, MapSet &, hipStream_t);
+template int build_maps<double>(const Group<double> &, const std::vector<int> &, MapSet &, hipStream_t, char *,
+                                size_t);
+template int build_maps<float>(const Group<float> &, const std::vector<int> &, MapSet &, hipStream_t, char *, size_t);
 
 template <typename T>
-int upload_table(Group<T> &G, hipStream_t s) {
+int upload_table(Group<T> &G, hipStream_t s, char *ext, size_t ext_cap) {
     int rc;
     const size_t bytes = G.sys.size() * sizeof(Sys<T>);
-    if ((rc = G.dev.ensure(std::max<size_t>(bytes, 256)))) return rc;
-    if (G.d_sys != (Sys<T> *)G.dev.ptr) G.drop_graph();
-    G.d_sys = (Sys<T> *)G.dev.ptr;
+    Sys<T> *dst = (Sys<T> *)ext;
+    if (ext) {
+        if (bytes > ext_cap) return MFIX_LEQ_ENOMEM;
+    } else {
+        if ((rc = G.dev.ensure(std::max<size_t>(bytes, 256)))) return rc;
+        dst = (Sys<T> *)G.dev.ptr;
+    }
+    if (G.d_sys != dst) G.drop_graph();
+    G.d_sys = dst;
     if (!bytes) return 0;
     hipError_t e = hipMemcpyAsync(G.d_sys, G.sys.data(), bytes, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     return e == hipSuccess ? 0 : hip_fail(e, "system table upload");
 }
-template int upload_table<double>(Group<double> &, hipStream_t);
-template int upload_table<float>(Group<float> &, hipStream_t);
+template int upload_table<double>(Group<double> &, hipStream_t, char *, size_t);
+template int upload_table<float>(Group<float> &, hipStream_t, char *, size_t);
 
 // ---- launch sequences -------------------------------------------------------------
 
@@ -607,12 +660,32 @@ template <typename T>
 static void line_sweeps(const Group<T> &G, const MapSet &M, int which, hipStream_t s) {
     Prof &pf = ctx().prof[1];
     for (int q = 0; q < M.nlpass; ++q) {
-        if (M.lmap[q].empty()) continue;
         for (int param : {1, 0}) {
-            pf.begin(s);
-            hipLaunchKernelGGL(k_line_color_sys<T>, dim3((unsigned)M.lmap[q].size()), dim3(192), 0, s, G.d_sys,
-                               M.d_lmap[q], param, which);
-            pf.end(s, 0.5 * M.cells * 13.0 * sizeof(T));
+            // one timed launch per colour and kernel family; the bytes are
+            // those of every LINE system of the table (a single solve: its own)
+            if (!M.lmap[q].empty()) {
+                pf.begin(s);
+                hipLaunchKernelGGL(k_line_color_sys<T>, dim3((unsigned)M.lmap[q].size()), dim3(192), 0, s, G.d_sys,
+                                   M.d_lmap[q], param, which);
+                pf.end(s, 0.5 * M.cells * 13.0 * sizeof(T));
+            }
+            if constexpr (sizeof(T) == sizeof(float)) {
+                for (int v = 0; v < kPcrVariants; ++v) {
+                    if (M.pmap[v][q].empty()) continue;
+                    const dim3 gs((unsigned)M.pmap[v][q].size()), bs(256);
+                    const Sys<float> *tab = (const Sys<float> *)G.d_sys;
+                    pf.begin(s);
+                    switch (v) {
+                    case 0: hipLaunchKernelGGL((k_line_pcr_sys<16, 2>), gs, bs, 0, s, tab, M.d_pmap[v][q], param, which); break;
+                    case 1: hipLaunchKernelGGL((k_line_pcr_sys<16, 4>), gs, bs, 0, s, tab, M.d_pmap[v][q], param, which); break;
+                    case 2: hipLaunchKernelGGL((k_line_pcr_sys<16, 8>), gs, bs, 0, s, tab, M.d_pmap[v][q], param, which); break;
+                    case 3: hipLaunchKernelGGL((k_line_pcr_sys<32, 8>), gs, bs, 0, s, tab, M.d_pmap[v][q], param, which); break;
+                    case 4: hipLaunchKernelGGL((k_line_pcr_sys<64, 5>), gs, bs, 0, s, tab, M.d_pmap[v][q], param, which); break;
+                    default: hipLaunchKernelGGL((k_line_pcr_sys<64, 8>), gs, bs, 0, s, tab, M.d_pmap[v][q], param, which); break;
+                    }
+                    pf.end(s, 0.5 * M.cells * 13.0 * sizeof(T));
+                }
+            }
         }
     }
     for (int id : M.line_seq) {
@@ -724,12 +797,16 @@ static int capture_chunk(Group<double> *G64, Group<float> *G32, int iters, hipSt
 // iterations (direct launches) or are a captured graph of 16, never more
 // than ITMAX.  Launches after a system is done return at once, so
 // overshooting is harmless, but each still dispatches its blocks.
-static int run_tables(Group<double> *G64, Group<float> *G32, int max_iters, hipStream_t s, bool graphs) {
+static int run_tables(Group<double> *G64, Group<float> *G32, int max_iters, hipStream_t s, bool graphs,
+                      int *d_count = nullptr) {
     Context &C = ctx();
     int rc;
-    if ((rc = C.misc.ensure(256 + 2 * sizeof(int) * kHostStates))) return rc;
-    int *d_cnt = (int *)C.misc.ptr;
-    int *d_flags64 = (int *)((char *)C.misc.ptr + 256);
+    if (!d_count) {
+        if ((rc = C.misc.ensure(kCountBytes))) return rc;
+        d_count = (int *)C.misc.ptr;
+    }
+    int *d_cnt = d_count;
+    int *d_flags64 = (int *)((char *)d_count + 256);
     int *d_flags32 = d_flags64 + kHostStates;
     if (G64 && G64->iter.ids.empty()) G64 = nullptr;
     if (G32 && G32->iter.ids.empty()) G32 = nullptr;
@@ -788,12 +865,12 @@ static int run_tables(Group<double> *G64, Group<float> *G32, int max_iters, hipS
 }
 
 template <typename T>
-int run_group(Group<T> &G, int max_iters, hipStream_t s, bool graphs) {
-    if constexpr (sizeof(T) == sizeof(double)) return run_tables(&G, nullptr, max_iters, s, graphs);
-    else return run_tables(nullptr, &G, max_iters, s, graphs);
+int run_group(Group<T> &G, int max_iters, hipStream_t s, bool graphs, int *d_count) {
+    if constexpr (sizeof(T) == sizeof(double)) return run_tables(&G, nullptr, max_iters, s, graphs, d_count);
+    else return run_tables(nullptr, &G, max_iters, s, graphs, d_count);
 }
-template int run_group<double>(Group<double> &, int, hipStream_t, bool);
-template int run_group<float>(Group<float> &, int, hipStream_t, bool);
+template int run_group<double>(Group<double> &, int, hipStream_t, bool, int *);
+template int run_group<float>(Group<float> &, int, hipStream_t, bool, int *);
 
 template <typename T>
 int read_states(const Group<T> &G, const std::vector<int> &ids, SolverState *h, hipStream_t s) {

@@ -210,6 +210,16 @@ def msolve_dev(pc: str, cmethod: str, a_m: DeviceArray, r: DeviceArray, z: Devic
     return check(rc, "msolve")
 
 
+def msolve32_dev(cmethod: str, a_m: DeviceArray, r: DeviceArray, z: DeviceArray) -> int:
+    """LINE preconditioner of the mixed inner solve on an 8-plane FP32
+    matrix (couplings at planes 0-2 and 4-6, A0 at plane 7)."""
+    assert a_m.dtype == np.float32 and r.dtype == np.float32 and z.dtype == np.float32
+    rc = _lib_().mfix_leq_msolve32_dev(cmethod.encode().ljust(4)[:4], a_m.ptr, r.ptr, z.ptr, None)
+    if rc == -106:
+        raise SolverAbort("DGTSV failed in a JK/IJ line sweep")
+    return check(rc, "msolve32")
+
+
 def dot_dev(x: DeviceArray, y: DeviceArray) -> float:
     out = C.c_double()
     check(_lib_().mfix_leq_dot_dev(x.ptr, y.ptr, C.byref(out), None), "dot")

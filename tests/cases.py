@@ -81,3 +81,27 @@ def n3_of(imax, jmax, kmax, no_k):
 
 def rng_vec(n, seed):
     return np.random.default_rng(seed).uniform(-1.0, 1.0, n)
+
+
+def oracle_spread(oracle_mod, og, pc, cmethod, tol, itmax, x0, a, b, noise, scales=(1e-15, -1e-15, 1e-14, -1e-14)):
+    """The oracle's own results when the R~ noise (model/leq_bicgs.f:298) is
+    perturbed at the level of one rounding error: the spread a change of
+    summation order alone can cause.  BiCGSTAB's trajectory is sensitive to
+    such perturbations (on a pressure-like system a 1e-14 change moves the
+    iteration count by up to 10 % and an unconverged iterate by percents),
+    so a solver that sums its dot products in another order is held to the
+    reference's own spread, not to one trajectory.  Returns [(x, result)]."""
+    out = []
+    for e in scales:
+        x, _, _, res = oracle_mod.leq_bicgs(og, pc, cmethod, tol, itmax, x0, a, b, noise * (1.0 + e))
+        out.append((x, res))
+    return out
+
+
+def iterations_ok(it_gpu, it_ref, spread):
+    """Within 10 % of the oracle's count, or within 10 % of the range its
+    own rounding-level perturbations give."""
+    if abs(it_gpu - it_ref) <= max(1, round(0.1 * it_ref)):
+        return True
+    its = [it_ref] + [r["iterations"] for _, r in spread]
+    return 0.9 * min(its) - 1 <= it_gpu <= 1.1 * max(its) + 1

@@ -9,6 +9,7 @@ import pytest
 
 from mfix_amd import leq, systems
 from mfix_amd.leq import DeviceArray
+from tests.cases import iterations_ok, oracle_spread
 
 pytestmark = pytest.mark.gpu
 
@@ -90,3 +91,51 @@ def test_each_call_keeps_its_grid(gpu):
     for (x_single, r), d, dx in zip(singles, done, held):
         assert d.ier == r.ier and d.stats["iterations"] == r.stats["iterations"]
         np.testing.assert_array_equal(dx.download(), x_single.download())
+
+
+@pytest.mark.parametrize("dims,no_k,cyc", [((20, 18, 16), False, (0, 0, 0)), ((40, 30, 1), True, (0, 0, 0)),
+                                           ((18, 16, 14), False, (1, 0, 1))])
+def test_flush_against_oracle(gpu, oracle_mod, dims, no_k, cyc):
+    """Each call of a flushed stage against the oracle's LEQ_BICGS on the
+    same inputs and R~ noise (a deferred call draws the library's default
+    stream, seed 0): FP64 calls give the same IER and iterations within 10 %
+    (or within the oracle's own rounding spread, tests.cases.oracle_spread);
+    a converged FP64 call's solution is within 1e-8, and one stopped at
+    ITMAX differs from the oracle's iterate no more than the oracle's
+    iterates differ among themselves under rounding-level perturbations
+    (x4); mixed calls that converge meet TOL on the FP64 true residual."""
+    im, jm, km = dims
+    print(f"sweep flush vs oracle {dims} no_k={no_k} cyc={cyc}", flush=True)
+    gpu.grid_init(im, jm, km, no_k=no_k, cyclic=tuple(bool(c) for c in cyc))
+    og = oracle_mod.grid(im, jm, km, no_k=no_k, cyclic=cyc)
+    sysdata = [_grid_system(kind, im, jm, km, no_k, 800 + q) for q, (_, _, kind, *_rest) in enumerate(STAGE)]
+    held, calls = [], []
+    for (vname, vno, kind, pc, sweep, itmax, tol, mixed), (a, b) in zip(STAGE, sysdata):
+        da, db, dx = DeviceArray.from_host(a), DeviceArray.from_host(b), DeviceArray.from_host(np.zeros(b.size))
+        held.append((dx, da, db))
+        calls.append(gpu.solve_lin_eq_deferred(vname, vno, dx, da, db, itmax, sweep, tol, pc, mixed=mixed))
+    done = gpu.sweep_flush()
+    for (vname, vno, kind, pc, sweep, itmax, tol, mixed), (a, b), d, (dx, da, db) in zip(STAGE, sysdata, done, held):
+        n3 = b.size
+        noise = systems.gen_uniform(n3, 0, 100)
+        xo, ao, bo, res = oracle_mod.leq_bicgs(og, pc, sweep, tol, itmax, np.zeros(n3), a, b, noise)
+        x = dx.download()
+        np.testing.assert_array_equal(da.download(), ao)
+        scale = max(np.max(np.abs(xo)), 1e-300)
+        if not mixed:
+            assert d.ier == res["ier"], (vname, d.stats, res)
+            spread = None
+            if not iterations_ok(d.stats["iterations"], res["iterations"], []):
+                spread = oracle_spread(oracle_mod, og, pc, sweep, tol, itmax, np.zeros(n3), a, b, noise)
+                assert iterations_ok(d.stats["iterations"], res["iterations"], spread), vname
+            if res["exit_reason"] == 0:
+                assert np.max(np.abs(x - xo)) <= 1e-8 * scale, vname
+            else:
+                spread = spread or oracle_spread(oracle_mod, og, pc, sweep, tol, itmax, np.zeros(n3), a, b, noise)
+                own = max(np.max(np.abs(xs - xo)) for xs, _ in spread)
+                assert np.max(np.abs(x - xo)) <= 4.0 * own + 1e-8 * scale, (vname, own / scale)
+        elif d.ier == 0:
+            r = bo - oracle_mod.matvec(og, ao, x)
+            assert np.sqrt(np.dot(r, r)) <= tol * res["rnorm0"] * 1.0000001, vname
+            if res["ier"] == 0:
+                assert np.max(np.abs(x - xo)) <= max(1e-8, 50.0 * tol) * scale, vname
