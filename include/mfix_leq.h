@@ -48,6 +48,7 @@ extern "C" {
 #define MFIX_LEQ_EDGTSV (-106)   /* DGTSV INFO>0 in a JK/IJ sweep: reference calls mfix_exit (leqsol_mod.f:991-996,1073-1078) */
 #define MFIX_LEQ_EBUSY (-107)    /* serving queue full */
 #define MFIX_LEQ_EMETHOD (-108)  /* SOLVE_LIN_EQ METHOD other than 2 (BiCGSTAB) */
+#define MFIX_LEQ_ETOOBIG (-109)  /* system larger than mfix_leq_bicgs_dev_noalloc's fixed workspace */
 
 /* ---- grid descriptor: replaces the compar/geometry/indices module state -- */
 typedef struct mfix_leq_grid {
@@ -117,6 +118,24 @@ int mfix_leq_bicgs_dev(const char *cmethod, double tol, const char *pc,
                        int itmax, double *d_var, double *d_a_m, double *d_b_m,
                        const mfix_leq_opts *opts, mfix_leq_stats *stats,
                        void *stream);
+
+/* mfix_leq_bicgs_dev without any allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under
+ * stream capture, tight serving loops, harnesses that forbid it).  The
+ * workspace, system table, block maps and poll words live in a fixed
+ * device array of MFIX_LEQ_NOALLOC_BYTES per GPU (held until the process
+ * ends; its address is taken when the library initialises on that GPU).
+ * Same arguments, results and bits as mfix_leq_bicgs_dev; no graph replays
+ * and no timing (stats->device_ms = 0).  Needs mfix_leq_init /
+ * mfix_leq_grid_init first (they allocate); returns MFIX_LEQ_ETOOBIG for a
+ * system whose workspace does not fit (FP64 DIAG/NONE up to about 1.3 M
+ * cells, LINE about 0.7 M).  Every call on a GPU uses that GPU's one array:
+ * calls that overlap on one GPU race and must be serialised by the caller. */
+#define MFIX_LEQ_NOALLOC_BYTES ((size_t)64 << 20)
+int mfix_leq_bicgs_dev_noalloc(const char *cmethod, double tol, const char *pc,
+                               int itmax, double *d_var, double *d_a_m,
+                               double *d_b_m, const mfix_leq_opts *opts,
+                               mfix_leq_stats *stats, void *stream);
 
 /* Mixed precision: FP32 inner BiCGSTAB (matvec, preconditioner, Krylov
  * vectors) inside FP64 iterative refinement; FP64 true residual, solution

@@ -24,6 +24,7 @@ const char *err_name(int code) {
     case MFIX_LEQ_EDGTSV: return "DGTSV failed in a JK/IJ line sweep";
     case MFIX_LEQ_EBUSY: return "serving queue full";
     case MFIX_LEQ_EMETHOD: return "only LEQ_METHOD=2 (BiCGSTAB) is provided";
+    case MFIX_LEQ_ETOOBIG: return "system larger than the no-allocation entry's fixed workspace (MFIX_LEQ_NOALLOC_BYTES)";
     default: return "unknown error";
     }
 }
@@ -110,7 +111,7 @@ int mfix_leq_iter_tot(int vno) {
 void mfix_leq_reset_iter_tot(void) { std::memset(ctx().iter_tot, 0, sizeof(ctx().iter_tot)); }
 
 const char *mfix_leq_error_string(int code) {
-    if (code == MFIX_LEQ_EHIP && ctx().last_msg[0]) return ctx().last_msg;
+    if ((code == MFIX_LEQ_EHIP || code == MFIX_LEQ_ETOOBIG) && ctx().last_msg[0]) return ctx().last_msg;
     return err_name(code);
 }
 

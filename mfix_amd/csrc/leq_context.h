@@ -115,6 +115,7 @@ struct Context {
     Group<float> g32;
     Group<double> b64;             // batch tables
     Group<float> b32;
+    Group<double> gna;             // table of the no-allocation entry (device side in its fixed array)
     Prof prof[kNumProf];
     Shape tuned[2][kSizeClasses];  // [fp32][size class]; bs == 0 means untuned
     int use_graphs = 1;            // capture chunks of iterations in HIP graphs
@@ -133,6 +134,9 @@ struct Context {
 
 Context &ctx();
 int hip_fail(hipError_t e, const char *what);
+// Device address of the no-allocation entry's fixed array on device `dev`
+// (the current device), taken once at set-up (leq_noalloc.hip).
+int noalloc_prepare(int dev);
 int parse_pc(const char *pc);
 DGrid make_dgrid(const mfix_leq_grid &h);
 double alg_bytes_per_cell(int pc, int sa, int sv, int sx);

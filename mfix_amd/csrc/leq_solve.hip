@@ -103,6 +103,8 @@ int Context::init(int dev) {
     hipError_t e = hipSetDevice(dev);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     device = dev;
+    int rc;
+    if ((rc = noalloc_prepare(dev))) return rc;
     e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
     if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
     e = hipHostMalloc((void **)&host_state, sizeof(SolverState) * kHostStates, hipHostMallocDefault);
