@@ -241,9 +241,10 @@ int mfix_leq_sweep_flush(mfix_leq_stats *stats, void *stream);
 int mfix_leq_sweep_pending(void);   /* calls queued since the last flush */
 void mfix_leq_sweep_discard(void);  /* drop the queued calls unsolved */
 
-/* Autotuner: pick the block shape of the fused kernels for one grid size;
- * results are cached per (n3 class, precision).  Returns the chosen
- * threads-per-block and lines-per-block. */
+/* Autotuner: pick the block shape of the fused kernels (timed on a DIAG
+ * solve) and the lines per block of the red-black line kernel (timed on a
+ * LINE solve) for one grid size; results are cached per (n3 class,
+ * precision).  Returns the chosen threads-per-block and lines-per-block. */
 int mfix_leq_autotune(int imax, int jmax, int kmax, int mixed, int *block,
                       int *lines_per_block);
 /* Pin the launch shape the autotuner would record for this grid size and
@@ -252,6 +253,20 @@ int mfix_leq_autotune(int imax, int jmax, int kmax, int mixed, int *block,
  * No counterpart in the reference. */
 int mfix_leq_set_shape(int imax, int jmax, int kmax, int mixed, int block_threads,
                        int cells_per_thread);
+/* Pin the lines per block (64, 32 or 16; 0 = the default for the size) of
+ * the red-black line kernel that LINE solves of this grid size and
+ * precision use: each block solves that many J-lines of one colour, one
+ * line per lane (the LEQ_MSOLVE/LEQ_IKSWEEP line solves,
+ * model/leqsol_mod.f:851-921).  The results do not depend on it.  No
+ * counterpart in the reference. */
+int mfix_leq_set_line_block(int imax, int jmax, int kmax, int mixed, int lines_per_block);
+/* Reliable-update trigger of the mixed-precision solves that do not set
+ * opts->inner_tol (batched and served ones included): an FP64 reliable
+ * update (true residual, solution accumulation) when the FP32 recursive
+ * residual fell by `delta` since the last one; 0 restores the default,
+ * 1e-4 below 2^23 cells and 0.1 from there on.  No counterpart in the
+ * reference. */
+int mfix_leq_set_mixed_delta(double delta);
 
 /* ---- residual norms of an assembled equation ---------------------------- */
 /* CALC_RESID_C / CALC_RESID_S / CALC_RESID_PP (model/calc_resid.f:16-178,

@@ -132,6 +132,8 @@ _SIGS = {
     "mfix_leq_set_graphs": (C.c_int, [C.c_int]),
     "mfix_leq_set_unit_diag": (C.c_int, [C.c_int]),
     "mfix_leq_set_shape": (C.c_int, [C.c_int] * 6),
+    "mfix_leq_set_line_block": (C.c_int, [C.c_int] * 5),
+    "mfix_leq_set_mixed_delta": (C.c_int, [C.c_double]),
     "mfix_assemble_pp_g_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "mfix_solve_pp_g_dev": (C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                                       C.c_int, C.c_char_p, C.c_double, C.c_char_p, C.c_int, C.c_void_p, C.c_void_p,

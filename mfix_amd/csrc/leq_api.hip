@@ -415,7 +415,25 @@ int mfix_leq_set_shape(int imax, int jmax, int kmax, int mixed, int block_thread
     if (block_threads != 256 && block_threads != 512 && block_threads != 1024) return MFIX_LEQ_EINVAL;
     if (cells_per_thread != 1 && cells_per_thread != 2 && cells_per_thread != 4) return MFIX_LEQ_EINVAL;
     const int64_t n3 = (int64_t)(imax + 2) * (jmax + 2) * (kmax + 2);
-    ctx().tuned[mixed ? 1 : 0][size_class(n3)] = Shape{block_threads, cells_per_thread, 64};
+    Shape &sh = ctx().tuned[mixed ? 1 : 0][size_class(n3)];
+    sh = Shape{block_threads, cells_per_thread, sh.lines_per_block};
+    return 0;
+}
+
+int mfix_leq_set_mixed_delta(double delta) {
+    if (!(delta >= 0.0 && delta < 1.0)) return MFIX_LEQ_EINVAL;
+    ctx().mixed_delta = delta;
+    return 0;
+}
+
+int mfix_leq_set_line_block(int imax, int jmax, int kmax, int mixed, int lines_per_block) {
+    if (imax < 1 || jmax < 1 || kmax < 1) return MFIX_LEQ_EINVAL;
+    if (lines_per_block != 0 && lines_per_block != 16 && lines_per_block != 32 && lines_per_block != 64)
+        return MFIX_LEQ_EINVAL;
+    const int64_t n3 = (int64_t)(imax + 2) * (jmax + 2) * (kmax + 2);
+    Shape &sh = ctx().tuned[mixed ? 1 : 0][size_class(n3)];
+    if (!sh.bs) sh = Shape{256, 1, 0};
+    sh.lines_per_block = lines_per_block;
     return 0;
 }
 

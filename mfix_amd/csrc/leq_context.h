@@ -53,17 +53,19 @@ struct Shape {
 };
 
 int size_class(int64_t n3);
+int default_lines_per_block(int64_t n3);
+double mixed_delta(int64_t n3);
 
 using Sweep = std::array<char, 4>;
 
 // Block maps of one launch sequence over the systems `ids` of a table:
 // stream kernels (smap), finish kernels (fmap), reliable updates (umap,
 // update_blocks() entries per system), batched red-black line
-// phases (lmap[pass]); LINE systems whose sweep string is not made of R/S
+// phases (lmap[lines-per-block variant][pass]); LINE systems whose sweep string is not made of R/S
 // passes are swept one by one (line_seq).
 struct MapSet {
     std::vector<int> ids;
-    std::vector<int2> smap, fmap, umap, lmap[kMaxRPasses];
+    std::vector<int2> smap, fmap, umap, lmap[kLineLpbVariants][kMaxRPasses];
     // FP32 tables: red-black passes of the several-lanes-per-line kernel,
     // one map per kernel variant (line length class) and pass
     std::vector<int2> pmap[kPcrVariants][kMaxRPasses];
@@ -74,7 +76,7 @@ struct MapSet {
     int n_line = 0, n_fused = 0;  // LINE systems / DIAG+NONE systems among ids
     DevBuf dev;
     int *d_ids = nullptr;
-    int2 *d_smap = nullptr, *d_fmap = nullptr, *d_umap = nullptr, *d_lmap[kMaxRPasses] = {};
+    int2 *d_smap = nullptr, *d_fmap = nullptr, *d_umap = nullptr, *d_lmap[kLineLpbVariants][kMaxRPasses] = {};
     int2 *d_pmap[kPcrVariants][kMaxRPasses] = {};
     uint64_t version = 0;         // bumped whenever the device copy changes
 };
@@ -99,6 +101,7 @@ struct Group {
 
 struct Context {
     bool initialised = false;
+    double mixed_delta = 0.0;      // mixed: reliable-update trigger, 0 = by size (mfix_leq_set_mixed_delta)
     bool have_grid = false;
     int device = 0;
     hipStream_t stream = nullptr;

@@ -19,6 +19,7 @@
 namespace mfix {
 
 constexpr int kNumXcd = 8;       // MI355X: 8 XCDs, blocks dispatched round-robin
+constexpr int kNumCus = 256;     // MI355X: 256 CUs (32 per XCD)
 constexpr int kFinishBlock = 1024;
 
 // Grid descriptor passed by value to every kernel.

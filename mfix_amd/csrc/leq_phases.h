@@ -589,30 +589,44 @@ __global__ void __launch_bounds__(kFinishBlock) k_finish_sys(const Sys<T> *__res
 }
 
 // Red-black line phase along J for every LINE system of a table (RSRS and
-// the 2-D R sweep).  Block b takes lines [map[b].y*64, +64) of colour
-// `param` of system map[b].x, numbered densely by color_line, with three
-// waves: lane l of wave 0 runs DGTSV on line l (the operations and order of
-// solve_line, so the result is bit-identical); waves 1 and 2 feed it.  Lines
-// along J are contiguous, so a loader wave reads rows in groups of 8 -
-// element k of its lane is row (lane & 7) of line (lane >> 3) + 8k, each
-// load instruction fetching 8 lines x 64 contiguous bytes - forms the
-// right-hand side (B minus the I/K neighbour terms, whose offsets are
-// constant along a J-line) and stages it in LDS.  A tile is 16 rows, 8 from
-// each loader, staged while wave 0 eliminates the previous tile: one memory
-// round trip per 16 rows (a 130-row line takes 9, where 8-row tiles took 17
-// and left the phase waiting on them).  Finished rows go to scratch in
-// [row][lane] order; the back solve reads them in groups of 8 rows, the next
-// group's loads in flight while the current one is solved, and wave 1 writes
-// the solution out through LDS, coalesced.  which = 0 preconditions P into
-// Phat, 1 S into Shat.
-template <typename T>
-__global__ void __launch_bounds__(192) k_line_color_sys(const Sys<T> *__restrict__ sys, const int2 *__restrict__ map,
+// the 2-D R sweep).  Block b takes lines [map[b].y*LPB, +LPB) of colour
+// `param` of system map[b].x, numbered densely by color_line, with five
+// waves: lane l < LPB of wave 0 runs DGTSV on line l (the operations and
+// order of solve_line, so the result is bit-identical); waves 1-4 feed it.
+// Lines along J are contiguous, so a loader wave reads rows in groups: a
+// tile is two groups of RL = 512/LPB rows, loader w takes row group
+// (w-1)/2 of line half (w-1)%2, and element k of its lane is row lane % RL
+// of line (half) + lane / RL + (LPB/8) k, each load instruction fetching
+// LPB/8 lines x RL contiguous rows; it forms the right-hand side (B minus
+// the I/K neighbour terms, whose offsets are constant along a J-line) and
+// stages it in LDS, while wave 0 eliminates the previous tile: one memory
+// round trip per tile (16 rows at 64 lines per block, 32 at 32, 64 at 16).
+// Finished rows go to scratch in [row][lane] order; the back solve reads
+// them in groups of 8 rows, the next group's loads in flight while the
+// current one is solved, and waves 1-2 write the solution out through LDS,
+// coalesced.  Fewer lines per block give more blocks (128^3: 8,450 lines per
+// colour are 133 blocks of 64 but 265 of 32); the launch shape picks LPB
+// per grid size.  Four loaders of four elements each keep the kernel at two
+// blocks per CU (a loader with eight elements needs 256 registers, one
+// block per CU).  which = 0 preconditions P into Phat, 1 S into Shat.
+template <typename T, int LPB>
+__global__ void __launch_bounds__(320) k_line_color_sys(const Sys<T> *__restrict__ sys, const int2 *__restrict__ map,
                                                         int param, int which) {
-    constexpr int R = 8;           // rows per loader wave and per back-solve group
-    constexpr int RT = 2 * R;      // rows per elimination tile
-    __shared__ T tl[2][4][RT][64]; // dl, d, du, bb: RT rows x 64 lines, double buffered
-    __shared__ T xs[R][64];        // solution rows staged for the stores
-    __shared__ int linfo[64];      // DGTSV INFO per line
+    static_assert(LPB == 64 || LPB == 32 || LPB == 16, "lines per block");
+    constexpr int RL = 512 / LPB;  // rows per row group
+    constexpr int LPI = 64 / RL;   // lines per load instruction
+    constexpr int K = 4;           // tile elements per loader lane
+    constexpr int RT = 2 * RL;     // rows per elimination tile
+    constexpr int RB = 8;          // rows per back-solve group
+    constexpr int KS = LPB / 16;   // solution elements per lane of each storing wave
+    // LDS rows padded so that a loader's staging store (RL rows x LPI lines)
+    // and a storing wave's read (8 rows x 8 lines) each touch every bank
+    // pair exactly twice, the minimum for 64 lanes of 8 bytes (unpadded,
+    // the rows share banks: 8- to 32-way conflicts)
+    constexpr int SL = LPB + LPI, SX = LPB + 8;
+    __shared__ T tl[2][4][RT][SL]; // dl, d, du, bb: RT rows x LPB lines, double buffered
+    __shared__ T xs[RB][SX];       // solution rows staged for the stores
+    __shared__ int linfo[LPB];      // DGTSV INFO per line
     const int2 e = map[blockIdx.x];
     if (e.x < 0) return;
     const Sys<T> &S = sys[e.x];
@@ -623,9 +637,11 @@ __global__ void __launch_bounds__(192) k_line_color_sys(const Sys<T> *__restrict
     const LineGeom L = line_geom(g, AX_J);
     const int n = L.n;
     const int64_t n3 = g.n3;
-    const int wave = threadIdx.x >> 6;  // 0: elimination and back solve; 1, 2: loads (1: stores)
-    const int ro = wave >= 1 ? R * (wave - 1) : 0;  // a loader's first row within a tile
+    const int wave = threadIdx.x >> 6;  // 0: elimination and back solve; 1-4: loads (1, 2: stores)
     const int lane = threadIdx.x & 63;
+    const bool mine = lane < LPB;       // wave 0: this lane solves line `lane` of the block
+    const int ro = wave >= 1 ? RL * ((wave - 1) >> 1) : 0;       // a loader's first row within a tile
+    const int lh = wave >= 1 ? (LPB / 2) * ((wave - 1) & 1) : 0;  // its first line
     const int do_k = g.do_k;
     constexpr int pd = DiagPlane<T>::value;
     const gptr<const T> a = to_global(S.a);
@@ -633,82 +649,71 @@ __global__ void __launch_bounds__(192) k_line_color_sys(const Sys<T> *__restrict
     const gptr<const T> pD = a + pd * n3, pJ_m = a + n3, pJ_p = a + 5 * n3;
     const gptr<const T> bm = to_global((const T *)(which == 0 ? S.p : S.s));
     const gptr<T> var = to_global(which == 0 ? S.phat : S.shat);
-    const gptr<T> sc = to_global(S.line_scratch + (int64_t)e.y * 4 * n * 64);
+    const gptr<T> sc = to_global(S.line_scratch + (int64_t)e.y * 4 * n * LPB);
     const uint32_t c1 = (uint32_t)g.c1, c2 = (uint32_t)g.c2;
-    const uint32_t pl = (uint32_t)n * 64;  // finished rows: 4 arrays of n rows x 64 lanes
+    const uint32_t pl = (uint32_t)n * LPB;  // finished rows: 4 arrays of n rows x LPB lanes
     int u = 0, v = 0;
-    const int vld = color_line(L.nu, L.nv, do_k ? 1 : 0, param, (int64_t)e.y * 64 + lane, u, v) ? 1 : 0;
-    const int rsub = lane & 7;
-    // loader view (wave 1): element k is row rsub of line (lane >> 3) + 8k,
-    // whose start cell and I/K neighbour offsets (constant along a J-line,
-    // set_increments.f:72-112) are kept per element
-    uint32_t jc[R], jim[R], jip[R], jkm[R], jkp[R];
-    int jok[R];
+    const int vld = mine && color_line(L.nu, L.nv, do_k ? 1 : 0, param, (int64_t)e.y * LPB + lane, u, v) ? 1 : 0;
+    const int rsub = lane % RL;
+    // loader view: element k is row rsub of line lh + lane / RL + LPI k, with
+    // its start cell (~0u: no line) and the I/K neighbour offsets, constant
+    // along a J-line (set_increments.f:72-112)
+    uint32_t jc[K], jim[K], jip[K], jkm[K], jkp[K];
 #pragma unroll
-    for (int k = 0; k < R; ++k) {
-        const int src = (lane >> 3) + 8 * k;
+    for (int k = 0; k < K; ++k) {
+        const int src = lh + lane / RL + LPI * k;
         const int iu = __shfl(u, src, 64), kv = __shfl(v, src, 64);
-        jok[k] = __shfl(vld, src, 64);
-        jc[k] = (uint32_t)iu * c1 + (uint32_t)kv * c2;
+        jc[k] = __shfl(vld, src, 64) ? (uint32_t)iu * c1 + (uint32_t)kv * c2 : ~0u;
         jim[k] = (uint32_t)(nb_minus(iu, g.ni, g.cyc_x) - iu) * c1;
         jip[k] = (uint32_t)(nb_plus(iu, g.ni, g.cyc_x) - iu) * c1;
         jkm[k] = do_k ? (uint32_t)(nb_minus(kv, g.nk, g.cyc_z) - kv) * c2 : 0u;
         jkp[k] = do_k ? (uint32_t)(nb_plus(kv, g.nk, g.cyc_z) - kv) * c2 : 0u;
     }
-    // Load tile rows [Q0, Q0+R) (every load of the tile issued before any is
-    // used) and stage them in LDS buffer BUF: the right-hand side in
-    // LEQ_IKSWEEP's order (leqsol_mod.f:891-902).
-    T tb[R], t1m[R], t1p[R], t2m[R], t2p[R], v1m[R], v1p[R], v2m[R], v2p[R], td[R], tdl[R], tdu[R];
-#define MFIX_LINE_LOAD(Q0)                                                                               \
-    _Pragma("unroll") for (int k = 0; k < R; ++k) {                                                      \
-        const int q = (Q0) + rsub;                                                                       \
-        const bool ok = jok[k] && q < n;                                                                 \
-        const uint32_t c = ok ? jc[k] + (uint32_t)q : jc[k]; /* in bounds even if loaded */                \
-        tb[k] = ok ? bm[c] : (T)0;                                                                       \
-        t1m[k] = ok ? pI_m[c] : (T)0;                                                                    \
-        t1p[k] = ok ? pI_p[c] : (T)0;                                                                    \
-        v1m[k] = ok ? var[c + jim[k]] : (T)0;                                                            \
-        v1p[k] = ok ? var[c + jip[k]] : (T)0;                                                            \
-        t2m[k] = ok && do_k ? pK_m[c] : (T)0;                                                            \
-        t2p[k] = ok && do_k ? pK_p[c] : (T)0;                                                            \
-        v2m[k] = ok && do_k ? var[c + jkm[k]] : (T)0;                                                    \
-        v2p[k] = ok && do_k ? var[c + jkp[k]] : (T)0;                                                    \
-        td[k] = ok ? pD[c] : (T)0;                                                                       \
-        tdl[k] = ok && q > 0 ? pJ_m[c] : (T)0;                                                           \
-        tdu[k] = ok && q < n - 1 ? pJ_p[c] : (T)0;                                                       \
-    }
-#define MFIX_LINE_STAGE(BUF)                                                                             \
-    _Pragma("unroll") for (int k = 0; k < R; ++k) {                                                      \
-        T bb;                                                                                            \
-        if (do_k) bb = tb[k] - t1m[k] * v1m[k] - t1p[k] * v1p[k] - t2m[k] * v2m[k] - t2p[k] * v2p[k];    \
-        else bb = tb[k] - t1m[k] * v1m[k] - t1p[k] * v1p[k];                                             \
-        const int j = (lane >> 3) + 8 * k;                                                               \
-        tl[BUF][0][ro + rsub][j] = tdl[k];                                                               \
-        tl[BUF][1][ro + rsub][j] = td[k];                                                                \
-        tl[BUF][2][ro + rsub][j] = tdu[k];                                                               \
-        tl[BUF][3][ro + rsub][j] = bb;                                                                   \
-    }
-#define MFIX_LINE_TILE(Q0, BUF) \
-    MFIX_LINE_LOAD(Q0)          \
-    MFIX_LINE_STAGE(BUF)
+    // Load the loader's rows [Q0, Q0+RL) of its elements (all loads issued
+    // before any is used) and stage them in LDS buffer BUF: the right-hand
+    // side in LEQ_IKSWEEP's order (leqsol_mod.f:891-902).
+    T tb[K], t1m[K], t1p[K], t2m[K], t2p[K], v1m[K], v1p[K], v2m[K], v2p[K], td[K], tdl[K], tdu[K];
+    auto tile = [&](int q0, int buf) __attribute__((always_inline)) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int q = q0 + rsub;
+            const bool ok = jc[k] != ~0u && q < n;
+            const uint32_t c = ok ? jc[k] + (uint32_t)q : 0u;  // in bounds even if loaded
+            tb[k] = ok ? bm[c] : (T)0;
+            t1m[k] = ok ? pI_m[c] : (T)0;
+            t1p[k] = ok ? pI_p[c] : (T)0;
+            v1m[k] = ok ? var[c + jim[k]] : (T)0;
+            v1p[k] = ok ? var[c + jip[k]] : (T)0;
+            t2m[k] = ok && do_k ? pK_m[c] : (T)0;
+            t2p[k] = ok && do_k ? pK_p[c] : (T)0;
+            v2m[k] = ok && do_k ? var[c + jkm[k]] : (T)0;
+            v2p[k] = ok && do_k ? var[c + jkp[k]] : (T)0;
+            td[k] = ok ? pD[c] : (T)0;
+            tdl[k] = ok && q > 0 ? pJ_m[c] : (T)0;
+            tdu[k] = ok && q < n - 1 ? pJ_p[c] : (T)0;
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            T bb;
+            if (do_k) bb = tb[k] - t1m[k] * v1m[k] - t1p[k] * v1p[k] - t2m[k] * v2m[k] - t2p[k] * v2p[k];
+            else bb = tb[k] - t1m[k] * v1m[k] - t1p[k] * v1p[k];
+            const int j = lh + lane / RL + LPI * k;
+            tl[buf][0][ro + rsub][j] = tdl[k];
+            tl[buf][1][ro + rsub][j] = td[k];
+            tl[buf][2][ro + rsub][j] = tdu[k];
+            tl[buf][3][ro + rsub][j] = bb;
+        }
+    };
     const int ntiles = (n + RT - 1) / RT;
-    if (wave >= 1) {
-        MFIX_LINE_TILE(ro, 0)
-    }
+    if (wave >= 1) tile(ro, 0);
     __syncthreads();
     T d0 = (T)0, du0 = (T)0, b0 = (T)0;
     int info = 0;
     for (int tt = 0; tt < ntiles; ++tt) {
         const int buf = tt & 1;
         if (wave >= 1) {
-            if (tt + 1 < ntiles) {
-                if (buf) {
-                    MFIX_LINE_TILE((tt + 1) * RT + ro, 0)
-                } else {
-                    MFIX_LINE_TILE((tt + 1) * RT + ro, 1)
-                }
-            }
-        } else {
+            if (tt + 1 < ntiles) tile((tt + 1) * RT + ro, buf ^ 1);
+        } else if (mine) {
             // DGTSV elimination of this lane's line over the tile
             // (DGTSV.f:107-163, the operations of solve_line in the same order)
             const int q0 = tt * RT;
@@ -760,7 +765,7 @@ __global__ void __launch_bounds__(192) k_line_color_sys(const Sys<T> *__restrict
                     fb = b1;
                     b1 = temp - fact * b1;
                 }
-                const uint32_t o = (uint32_t)i * 64 + lane;
+                const uint32_t o = (uint32_t)i * LPB + lane;
                 sc[o] = fd;
                 sc[pl + o] = fu;
                 sc[2 * pl + o] = fu2;
@@ -772,28 +777,32 @@ __global__ void __launch_bounds__(192) k_line_color_sys(const Sys<T> *__restrict
         }
         __syncthreads();
     }
-#undef MFIX_LINE_TILE
-#undef MFIX_LINE_LOAD
-#undef MFIX_LINE_STAGE
-    if (wave == 0) {
+    if (wave == 0 && mine) {
         if (!info && d0 == (T)0) info = n;  // DGTSV.f:160-163
         linfo[lane] = info;
     }
     __syncthreads();
-    int jinfo[R];
+    // storing view (waves 1 and 2): element k is row lane & 7 of line
+    // (lane >> 3) + 8 (KS (wave - 1) + k); a line whose elimination hit a zero
+    // pivot keeps its values (leqsol_mod.f:910-914)
+    uint32_t sjc[KS];
 #pragma unroll
-    for (int k = 0; k < R; ++k) jinfo[k] = linfo[(lane >> 3) + 8 * k];
-    // back solve (DGTSV.f:233-246) in groups of R rows from the top row
+    for (int k = 0; k < KS; ++k) {
+        const int src = (lane >> 3) + 8 * (KS * (wave == 2 ? 1 : 0) + k);
+        const int su = __shfl(u, src, 64), sv = __shfl(v, src, 64);
+        sjc[k] = __shfl(vld, src, 64) && !linfo[src] ? (uint32_t)su * c1 + (uint32_t)sv * c2 : ~0u;
+    }
+    // back solve (DGTSV.f:233-246) in groups of RB rows from the top row
     // down; rows n-2 .. 0 come from scratch (row n-1 is still in registers)
-    T x1 = b0 / d0, x2 = (T)0;
-    T gD[R], gU[R], gU2[R], gB[R];
+    T x1 = mine ? b0 / d0 : (T)0, x2 = (T)0;
+    T gD[RB], gU[RB], gU2[RB], gB[RB];
     auto fetch = [&](int hi) __attribute__((always_inline)) {
-        const int lo = hi - R + 1 > 0 ? hi - R + 1 : 0;
+        const int lo = hi - RB + 1 > 0 ? hi - RB + 1 : 0;
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
+        for (int r = 0; r < RB; ++r) {
             const int q = hi - r;
             if (q >= lo && q <= n - 2) {
-                const uint32_t o = (uint32_t)q * 64 + lane;
+                const uint32_t o = (uint32_t)q * LPB + lane;
                 gD[r] = sc[o];
                 gU[r] = sc[pl + o];
                 gU2[r] = sc[2 * pl + o];
@@ -801,13 +810,13 @@ __global__ void __launch_bounds__(192) k_line_color_sys(const Sys<T> *__restrict
             }
         }
     };
-    if (wave == 0) fetch(n - 1);
-    for (int hi = n - 1; hi >= 0; hi -= R) {
-        const int lo = hi - R + 1 > 0 ? hi - R + 1 : 0;
-        if (wave == 0) {
-            T cD[R], cU[R], cU2[R], cB[R];
+    if (wave == 0 && mine) fetch(n - 1);
+    for (int hi = n - 1; hi >= 0; hi -= RB) {
+        const int lo = hi - RB + 1 > 0 ? hi - RB + 1 : 0;
+        if (wave == 0 && mine) {
+            T cD[RB], cU[RB], cU2[RB], cB[RB];
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
+            for (int r = 0; r < RB; ++r) {
                 cD[r] = gD[r];
                 cU[r] = gU[r];
                 cU2[r] = gU2[r];
@@ -815,7 +824,7 @@ __global__ void __launch_bounds__(192) k_line_color_sys(const Sys<T> *__restrict
             }
             if (lo > 0) fetch(lo - 1);  // next group in flight while this one is solved
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
+            for (int r = 0; r < RB; ++r) {
                 const int q = hi - r;
                 if (q >= lo) {
                     T x;
@@ -835,18 +844,25 @@ __global__ void __launch_bounds__(192) k_line_color_sys(const Sys<T> *__restrict
             }
         }
         __syncthreads();
-        if (wave == 1) {
+        if (wave == 1 || wave == 2) {
 #pragma unroll
-            for (int k = 0; k < R; ++k) {
-                const int q = lo + rsub;
-                // a line whose elimination hit a zero pivot keeps its values
-                // (leqsol_mod.f:910-914)
-                if (jok[k] && !jinfo[k] && q <= hi) var[jc[k] + (uint32_t)q] = xs[rsub][(lane >> 3) + 8 * k];
+            for (int k = 0; k < KS; ++k) {
+                const int q = lo + (lane & 7);
+                const int j = (lane >> 3) + 8 * (KS * (wave - 1) + k);
+                if (sjc[k] != ~0u && q <= hi) var[sjc[k] + (uint32_t)q] = xs[lane & 7][j];
             }
         }
         __syncthreads();
     }
 }
+
+constexpr int kLineBlock = 320;  // threads of k_line_color_sys: one solving wave, four loaders
+
+// Lines-per-block variants of k_line_color_sys (a launch shape picks one per
+// grid size).
+constexpr int kLineLpbVariants = 3;
+__host__ __device__ constexpr int line_lpb_of(int var) { return var == 0 ? 64 : var == 1 ? 32 : 16; }
+__host__ __device__ inline int line_lpb_variant(int lpb) { return lpb >= 64 ? 0 : lpb >= 32 ? 1 : 2; }
 
 // Red-black line phase along J for the FP32 inner solve of the mixed path:
 // several lanes per line.  The FP64 kernel above must repeat DGTSV's

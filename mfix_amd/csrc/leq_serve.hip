@@ -140,7 +140,7 @@ int admit(mfix_leq_server *S, Group<T> &G, Req &q, int slot, hipStream_t s) {
     d.itmax = q.r.itmax;
     d.minimize = 0;
     d.icheck = 1;
-    d.delta = 0.1;
+    d.delta = mixed_delta(g.n3);
     d.max_updates = 0;
     d.x = x;
     d.a = a;

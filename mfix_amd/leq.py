@@ -253,6 +253,18 @@ def set_graphs(on: bool) -> None:
     check(_lib_().mfix_leq_set_graphs(int(on)), "set_graphs")
 
 
+def set_line_block(imax: int, jmax: int, kmax: int, lines_per_block: int, mixed: bool = False) -> None:
+    """Lines per block (64, 32, 16; 0 = size default) of the red-black line
+    kernel for LINE solves of this grid size and precision."""
+    check(_lib_().mfix_leq_set_line_block(imax, jmax, kmax, int(mixed), lines_per_block), "set_line_block")
+
+
+def set_mixed_delta(delta: float) -> None:
+    """Reliable-update trigger of mixed solves that set none: the FP32
+    recursive residual fell by `delta` (0 = the size default)."""
+    check(_lib_().mfix_leq_set_mixed_delta(float(delta)), "set_mixed_delta")
+
+
 def set_unit_diag(on: bool) -> None:
     """Mixed DIAG solves negate instead of reading 1/A0 when every 1/A0 is -1
     (default on); results are bit-identical either way."""

