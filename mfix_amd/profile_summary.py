@@ -4,9 +4,13 @@ counter collections) into per-kernel averages.
     python -m mfix_amd.profile_summary DIR [DIR ...]
 
 For every ``*kernel_stats.csv`` found under a DIR: calls, total and average
-duration per kernel.  For every ``*counter_collection.csv``: the mean value
-per dispatch of each counter per kernel (FETCH_SIZE / WRITE_SIZE are in KiB
-per dispatch).
+duration per kernel.  For every ``*kernel_trace.csv``: per kernel and grid
+size, the launches that did work apart from those that returned at once (a
+solve's launches after it converged, whose blocks only read the done flag):
+a launch counts as working when it lasted at least a quarter of the longest
+launch of the same kernel and grid.  For every ``*counter_collection.csv``:
+the mean value per dispatch of each counter per kernel (FETCH_SIZE /
+WRITE_SIZE are in KiB per dispatch).
 """
 from __future__ import annotations
 
@@ -40,6 +44,23 @@ def kernel_stats(path: Path) -> list[tuple]:
     return out
 
 
+def trace_working(path: Path) -> list[tuple]:
+    durs = defaultdict(list)
+    with path.open() as f:
+        for row in csv.DictReader(f):
+            name = _col(row, "Kernel_Name", "KernelName", "Name")
+            grid = _col(row, "Grid_Size_X", "Grid_Size", "Grid_SizeX") or "?"
+            t0, t1 = _col(row, "Start_Timestamp"), _col(row, "End_Timestamp")
+            durs[(name, grid)].append(int(t1) - int(t0))
+    out = []
+    for (name, grid), v in durs.items():
+        cut = 0.25 * max(v)
+        w = sorted(x for x in v if x >= cut)
+        out.append((name, grid, len(v), len(w), sum(w), sum(w) / len(w), w[len(w) // 2]))
+    out.sort(key=lambda t: -t[4])
+    return out
+
+
 def counters(path: Path) -> dict:
     acc = defaultdict(lambda: [0, 0.0])
     with path.open() as f:
@@ -61,6 +82,11 @@ def main(argv: list[str]) -> int:
             print(f"{'calls':>8} {'total_ms':>10} {'avg_us':>9}  kernel")
             for name, calls, total, avg in kernel_stats(p)[:25]:
                 print(f"{calls:8d} {total / 1e6:10.2f} {avg / 1e3:9.2f}  {name}")
+        for p in sorted(root.rglob("*kernel_trace.csv")):
+            print(f"# working launches (>= 1/4 of the longest of the same kernel and grid): {p}")
+            print(f"{'grid':>10} {'calls':>7} {'working':>8} {'total_ms':>10} {'avg_us':>9} {'median_us':>10}  kernel")
+            for name, grid, n, nw, tot, avg, med in trace_working(p)[:30]:
+                print(f"{grid:>10} {n:7d} {nw:8d} {tot / 1e6:10.2f} {avg / 1e3:9.2f} {med / 1e3:10.2f}  {name}")
         for p in sorted(root.rglob("*counter_collection.csv")):
             print(f"# counters: {p}")
             print(f"{'dispatches':>10} {'mean':>14}  counter  kernel")
