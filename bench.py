@@ -667,8 +667,8 @@ def main(argv=None):
     value = args.steps / total
     fp64_value = len(fp64_times) / sum(fp64_times) if fp64_times else None
 
-    # dominant kernel: the first FP32 stencil pass k_matvec_dots2<float, 0>
-    # (V = A*(P/A0) and <Rtilde,V>, two cells per thread): 6 off-diagonal planes + row excess
+    # dominant kernel: the first FP32 stencil pass k_matvec_dots4<BS, 0>
+    # (V = A*(P/A0) and <Rtilde,V>, four cells per thread): 6 off-diagonal planes + row excess
     # + operand + dot partner + result = 10*4 = 40 B/cell (HIP events in this
     # run; A0 = -1 on every unit-scaled row of this system, so 1/A0 is not read).
     # Only launches that worked count: the ones issued after a solve was done
@@ -733,7 +733,7 @@ def main(argv=None):
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_matvec_dots2<float, 0> (FP32 stencil V = A*(P/A0) + <Rtilde,V>)",
+            "kernel": "k_matvec_dots4<BS, 0> (FP32 stencil V = A*(P/A0) + <Rtilde,V>)",
             "achieved": achieved / 1e9,
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",

@@ -200,6 +200,11 @@ template <typename E, int N, typename P>
 __device__ __forceinline__ vec_t<E, N> ldv(P p) {
     return __builtin_nontemporal_load((const __attribute__((address_space(1))) vec_t<E, N> *)p);
 }
+// Plain (cached) vector load, for values other threads read again.
+template <typename E, int N, typename P>
+__device__ __forceinline__ vec_t<E, N> ldv_keep(P p) {
+    return *(const __attribute__((address_space(1))) vec_t<E, N> *)p;
+}
 template <typename E, int N, typename P>
 __device__ __forceinline__ void stv(P p, vec_t<E, N> v) {
     *(__attribute__((address_space(1))) vec_t<E, N> *)p = v;

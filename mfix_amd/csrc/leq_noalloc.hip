@@ -47,7 +47,7 @@ size_t map_bytes(const DGrid &g, const Part &P) {
     b += up256(e * (size_t)xcd_grid(P.nblk));                        // smap
     b += up256(e * kFinishBlocks);                                   // fmap
     b += up256(e * (size_t)update_blocks(P.nblk));                   // umap
-    b += kMaxRPasses * up256(e * (size_t)((lines + 15) / 16));       // lmap of the variant in use
+    b += kMaxRPasses * up256(e * (size_t)((lines + 15) / 16 + kNumXcd));  // lmap of the variant in use
     b += (kLineLpbVariants - 1) * kMaxRPasses * 256;                 // the other variants' empty maps
     b += kPcrVariants * kMaxRPasses * 256;                           // empty FP32 maps
     return b;

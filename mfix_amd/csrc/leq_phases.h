@@ -245,19 +245,37 @@ __global__ void __launch_bounds__(BS) k_matvec_dots(const Sys<T> *__restrict__ s
     }
 }
 
-// k_matvec_dots with two cells per thread, BS/2 threads over the same BS-cell
-// logical blocks: thread t takes cells t and t + BS/2 and issues the loads of
-// both rows before it waits on either, so a wave keeps twice the loads in
-// flight for one pass through the block's set-up.  FP32 inner solve only (a
-// thread sums its two cells before the wave tree, so the partials round
-// differently from the one-cell kernel).
-template <int BS, typename T, int MODE>
-__global__ void __launch_bounds__(BS / 2) k_matvec_dots2(const Sys<T> *__restrict__ sys,
+// k_matvec_dots for the FP32 inner solve with four consecutive cells per
+// thread (BS/4 threads over a BS-cell logical block) and 16-byte accesses:
+// the seven coefficient planes, the dot partner and the result move as one
+// dwordx4 per lane each, and the operand as five (at c, c -+ c1, c -+ c2:
+// every I and K neighbour of the thread's four cells lies in the same
+// relative position of the neighbouring row and plane).  The J neighbours
+// come from the adjacent lanes' groups by lane shuffles; lanes 0 and 63 load
+// the one value past their wave's run.  Per four cells that is 14 memory
+// instructions where one cell per thread needs 16 per cell; one cell per
+// thread streamed at 4.8-5.1 TB/s against 6.3-6.7 TB/s for the update
+// passes, whose lanes already move 16 bytes.  A cell on the I or J edge of
+// the grid takes itself as the clamped neighbour (set_increments.f:72-112),
+// as neighbours_fast does, by a select on its (i, j).  Groups on the first
+// or last K plane, past the end of the grid, of a 2-D grid or of a grid
+// with a cyclic axis (whose wrapped neighbours are not at fixed offsets)
+// take the per-cell path (Stencil<float>::row).  The row arithmetic is
+// stencil_combine_diff either way, so every row is bit-identical to the
+// per-cell path's; a thread adds its four cells before the wave tree.
+// Measured at 256^3 (the void-fraction pressure system, 256-cell blocks):
+// 133.5 / 127.8 us per pass against 133.7 / 129.3 us for two cells per
+// thread with 4-byte accesses, so the instruction count was not the limit;
+// both move 5.6 TB/s of measured HBM traffic.  The seven planes of a cell
+// stored together (one 32-byte record per cell, one stream instead of
+// seven) ran 290 / 279 us: each record load touched a cache line per lane.
+template <int BS, int MODE>
+__global__ void __launch_bounds__(BS / 4) k_matvec_dots4(const Sys<float> *__restrict__ sys,
                                                           const int2 *__restrict__ map) {
-    constexpr int NT = BS / 2;
-    __shared__ double lds[(NT / 64) * 2];
+    constexpr int NT = BS / 4;
+    __shared__ double lds[(NT / 64 > 0 ? NT / 64 : 1) * 2];
     __shared__ unsigned arrived;
-    SYS_BLOCK(T)
+    SYS_BLOCK(float)
     auto *__restrict__ st = to_global(S.st);
     if (st->done) return;
     if (MODE == 1 && st->s_exit) return;
@@ -265,38 +283,104 @@ __global__ void __launch_bounds__(BS / 2) k_matvec_dots2(const Sys<T> *__restric
     partial_begin<NT>(&arrived);
     double v[2] = {0.0, 0.0};
     const bool all = MODE == 1 ? (g.dot_all && !st->minimize) : g.dot_all;
+    const int64_t n3 = g.n3;
     auto *__restrict__ a = to_global(S.a);
-    const auto a0 = a + (int64_t)DiagPlane<T>::value * g.n3;
+    const auto a0p = a + (int64_t)DiagPlane<float>::value * n3;
     auto *__restrict__ y = MODE == 0 ? to_global(S.v) : to_global(S.t);
     auto *__restrict__ w = MODE == 0 ? to_global(S.rt) : to_global(S.s);
-    const auto xin = MODE == 0 ? (S.pc == PC_LINE ? to_global(S.phat) : to_global(S.p)) : (S.pc == PC_LINE ? to_global(S.shat) : to_global(S.s));
-    const int64_t lo = blk * P.chunk, end = lo + P.chunk < P.n ? lo + P.chunk : P.n;
-    auto body = [&](auto X) {
-      for (int64_t base = lo; base < end; base += BS) {
-        const int64_t ca = base + threadIdx.x, cq = ca + NT;
-        const bool va = ca < end, vb = cq < end;
-        const int64_t c0 = va ? ca : base, c1 = vb ? cq : c0;  // in bounds even when not stored
-        const Nb n0 = neighbours_fast(g, c0), n1 = neighbours_fast(g, c1);
-        const T w0 = ld_stream(&w[c0]), w1 = ld_stream(&w[c1]);
-        const T y0 = Stencil<T>::row(g, a, X, c0, n0);
-        const T y1 = Stencil<T>::row(g, a, X, c1, n1);
-        if (va) y[c0] = y0;
-        if (vb) y[c1] = y1;
-        if (va && (all || n0.interior)) {
-            const double yd = (double)y0, wd = (double)w0;
-            v[0] += (MODE == 0) ? wd * yd : yd * wd;
-            if (MODE == 1) v[1] += yd * yd;
-        }
-        if (vb && (all || n1.interior)) {
-            const double yd = (double)y1, wd = (double)w1;
-            v[0] += (MODE == 0) ? wd * yd : yd * wd;
-            if (MODE == 1) v[1] += yd * yd;
-        }
-      }
+    const auto xin = MODE == 0 ? (S.pc == PC_LINE ? to_global(S.phat) : to_global(S.p))
+                               : (S.pc == PC_LINE ? to_global(S.shat) : to_global(S.s));
+    const int op = S.pc != PC_DIAG ? OP_PLAIN : (st->nonunit_diag ? OP_DIAG : OP_NEG);
+    const int64_t c0 = blk * P.chunk + 4 * (int64_t)threadIdx.x;
+    const int64_t c1 = g.c1, c2 = g.c2;
+    const int lane = threadIdx.x & 63;
+    // fast path: all four cells exist and lie off the first and last K
+    // planes (so every load below is in bounds), on a 3-D grid without
+    // cyclic axes
+    const bool fast = g.do_k && !(g.cyc_x | g.cyc_y | g.cyc_z) && c0 >= c2 && c0 + 3 + c2 < n3;
+    f4v X = 0.f, XIm = 0.f, XIp = 0.f, XKm = 0.f, XKp = 0.f, A[7], W = 0.f;
+    auto xop = [&](int64_t q) -> float {
+        const float e = xin[q];
+        return op == OP_DIAG ? diag_apply(e, (float)a0p[q]) : op == OP_NEG ? -e : e;
     };
-    if (S.pc == PC_DIAG && st->nonunit_diag) body(Operand<T, OP_DIAG, gptr<const T>>{xin, a0});
-    else if (S.pc == PC_DIAG) body(Operand<T, OP_NEG, gptr<const T>>{xin, nullptr});
-    else body(Operand<T, OP_PLAIN, gptr<const T>>{xin, nullptr});
+    if (fast) {
+        // the operand with plain loads: its values are read again as other
+        // cells' I and K neighbours, from L2
+        X = ldv_keep<float, 4>(xin + c0);
+        XIm = ldv_keep<float, 4>(xin + c0 - c1);
+        XIp = ldv_keep<float, 4>(xin + c0 + c1);
+        XKm = ldv_keep<float, 4>(xin + c0 - c2);
+        XKp = ldv_keep<float, 4>(xin + c0 + c2);
+#pragma unroll
+        for (int p = 0; p < 7; ++p) A[p] = ldv<float, 4>(a + (int64_t)p * n3 + c0);
+        W = ldv<float, 4>(w + c0);
+        if (op == OP_DIAG) {
+            X = X * ldv_keep<float, 4>(a0p + c0);
+            XIm = XIm * ldv_keep<float, 4>(a0p + c0 - c1);
+            XIp = XIp * ldv_keep<float, 4>(a0p + c0 + c1);
+            XKm = XKm * ldv_keep<float, 4>(a0p + c0 - c2);
+            XKp = XKp * ldv_keep<float, 4>(a0p + c0 + c2);
+        } else if (op == OP_NEG) {
+            X = -X;
+            XIm = -XIm;
+            XIp = -XIp;
+            XKm = -XKm;
+            XKp = -XKp;
+        }
+    }
+    // J neighbours across a group's edges: the adjacent lane's group when
+    // that lane is on the fast path too, else one load (every lane takes part
+    // in the shuffles)
+    float xl = __shfl_up(X.w, 1, 64), xr = __shfl_down(X.x, 1, 64);
+    const int fl = __shfl_up((int)fast, 1, 64), fr = __shfl_down((int)fast, 1, 64);
+    if (fast && (lane == 0 || !fl)) xl = xop(c0 - 1);
+    if (fast && (lane == 63 || !fr)) xr = xop(c0 + 4);
+    if (fast) {
+        int i, j, k;
+        cell_ijk(g, c0, i, j, k);
+        f4v Y;
+#pragma unroll
+        for (int l = 0; l < 4; ++l) {
+            const float xc = X[l];
+            const float xjm = j == 0 ? xc : (l == 0 ? xl : X[l - 1]);
+            const float xjp = j == g.nj - 1 ? xc : (l == 3 ? xr : X[l + 1]);
+            const float xim = i == 0 ? xc : XIm[l];
+            const float xip = i == g.ni - 1 ? xc : XIp[l];
+            const float yy = stencil_combine_diff(true, A[0][l], A[1][l], A[2][l], A[3][l], A[4][l], A[5][l], A[6][l],
+                                                  XKm[l], xim, xjm, xc, xjp, xip, XKp[l]);
+            Y[l] = yy;
+            const bool interior = i > 0 && i < g.ni - 1 && j > 0 && j < g.nj - 1;  // k is interior here
+            if (all || interior) {
+                const double yd = (double)yy, wd = (double)W[l];
+                v[0] += (MODE == 0) ? wd * yd : yd * wd;
+                if (MODE == 1) v[1] += yd * yd;
+            }
+            if (++j == g.nj) {
+                j = 0;
+                if (++i == g.ni) i = 0;
+            }
+        }
+        stv<float, 4>(y + c0, Y);
+    } else {
+        // per-cell path
+        const int64_t end = blk * P.chunk + P.chunk < P.n ? blk * P.chunk + P.chunk : P.n;
+        auto cells = [&](auto X) {
+            for (int64_t c = c0; c < c0 + 4 && c < end; ++c) {
+                const Nb nb = neighbours_fast(g, c);
+                const float wc = ld_stream(&w[c]);
+                const float yy = Stencil<float>::row(g, a, X, c, nb);
+                y[c] = yy;
+                if (all || nb.interior) {
+                    const double yd = (double)yy, wd = (double)wc;
+                    v[0] += (MODE == 0) ? wd * yd : yd * wd;
+                    if (MODE == 1) v[1] += yd * yd;
+                }
+            }
+        };
+        if (op == OP_DIAG) cells(Operand<float, OP_DIAG, gptr<const float>>{xin, a0p});
+        else if (op == OP_NEG) cells(Operand<float, OP_NEG, gptr<const float>>{xin, nullptr});
+        else cells(Operand<float, OP_PLAIN, gptr<const float>>{xin, nullptr});
+    }
     if (MODE == 0) {
         double v1[1] = {v[0]};
         block_partial_last<NT, 1>(v1, to_global(S.partials), P.nblk, blk, lds, &arrived);

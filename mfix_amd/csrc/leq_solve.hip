@@ -228,6 +228,20 @@ int size_class(int64_t n3) {
     default: { constexpr int BSV = 256; __VA_ARGS__; } break; \
     }
 
+// Map entries of system `id`'s nb logical blocks in XCD slab order: slot j
+// of an 8-aligned range runs on XCD j % 8 and holds the logical block that
+// gives each XCD one contiguous run of the system (padding slots: -1).
+static void push_slab_map(std::vector<int2> &out, int id, int64_t nb) {
+    const int64_t per = (nb + kNumXcd - 1) / kNumXcd;
+    const int64_t full = nb - (per - 1) * kNumXcd;
+    for (int64_t j = 0; j < per * kNumXcd; ++j) {
+        const int64_t x = j % kNumXcd, r = j / kNumXcd;
+        const bool valid = r < (x < full ? per : per - 1);
+        const int64_t start = x < full ? x * per : full * per + (x - full) * (per - 1);
+        out.push_back(valid ? make_int2(id, (int)(start + r)) : make_int2(-1, -1));
+    }
+}
+
 static inline dim3 grid_of(const Part &P) { return dim3((unsigned)xcd_grid(P.nblk)); }
 static inline size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
@@ -362,7 +376,7 @@ int run_msolve_line_rb(const DGrid &g, const char *cmethod, const T *a, const T 
     const size_t o_st = align256(sizeof(Sys<T>));
     const size_t o_map = o_st + align256(sizeof(SolverState));
     int rc;
-    if ((rc = C.line1.ensure(o_map + sizeof(int2) * (size_t)lb))) return rc;
+    if ((rc = C.line1.ensure(o_map + sizeof(int2) * (size_t)(lb + kNumXcd)))) return rc;
     char *base = (char *)C.line1.ptr;
     Sys<T> S = {};
     S.g = g;
@@ -372,8 +386,8 @@ int run_msolve_line_rb(const DGrid &g, const char *cmethod, const T *a, const T 
     S.phat = z;
     S.line_scratch = scratch;
     S.st = (SolverState *)(base + o_st);
-    std::vector<int2> map((size_t)lb);
-    for (int64_t b = 0; b < lb; ++b) map[(size_t)b] = make_int2(0, (int)b);
+    std::vector<int2> map;
+    push_slab_map(map, 0, lb);
     hipError_t e = hipMemcpyAsync(base, &S, sizeof(S), hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(base + o_map, map.data(), sizeof(int2) * map.size(), hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(z, r, sizeof(T) * (size_t)g.n3, hipMemcpyDeviceToDevice, s);
@@ -382,7 +396,7 @@ int run_msolve_line_rb(const DGrid &g, const char *cmethod, const T *a, const T 
     Prof &pf = C.prof[1];
     const Sys<T> *tab = (const Sys<T> *)base;
     const int2 *mp = (const int2 *)(base + o_map);
-    const dim3 gs((unsigned)lb);
+    const dim3 gs((unsigned)map.size());
     for (int q = 0; q < np; ++q)
         for (int param : {1, 0}) {
             pf.begin(s);
@@ -541,14 +555,7 @@ int build_maps(const Group<T> &G, const std::vector<int> &ids, MapSet &M, hipStr
         // stream kernels: an 8-aligned range whose slot j holds the logical
         // block that puts one contiguous slab of the system on each XCD
         const int64_t nb = S.P.nblk;
-        const int64_t per = (nb + kNumXcd - 1) / kNumXcd;
-        const int64_t full = nb - (per - 1) * kNumXcd;
-        for (int64_t j = 0; j < per * kNumXcd; ++j) {
-            const int64_t x = j % kNumXcd, r = j / kNumXcd;
-            const bool valid = r < (x < full ? per : per - 1);
-            const int64_t start = x < full ? x * per : full * per + (x - full) * (per - 1);
-            smap.push_back(valid ? make_int2(id, (int)(start + r)) : make_int2(-1, -1));
-        }
+        push_slab_map(smap, id, nb);
         for (int f = 0; f < S.nfin; ++f) fmap.push_back(make_int2(id, f));
         for (int64_t u = 0; u < update_blocks(nb); ++u) umap.push_back(make_int2(id, (int)u));
         if (S.pc == PC_LINE) {
@@ -564,9 +571,14 @@ int build_maps(const Group<T> &G, const std::vector<int> &ids, MapSet &M, hipStr
                 const int lv = line_lpb_variant(ctx().shape_for(S.g.n3, sizeof(T) == sizeof(float)).lines_per_block);
                 const int lpb = var >= 0 ? line_pcr_lpb(var) : line_lpb_of(lv);
                 const int64_t lb = (lines + lpb - 1) / lpb;
-                for (int q = 0; q < np; ++q)
-                    for (int64_t b = 0; b < lb; ++b)
-                        (var >= 0 ? pmap[var][q] : lmap[lv][q]).push_back(make_int2(id, (int)b));
+                // the same XCD slab order as the stream kernels: XCD x takes a
+                // contiguous run of line blocks, so the I/K neighbour lines its
+                // blocks read (the other colour's, about two blocks away along
+                // K) come through one L2
+                for (int q = 0; q < np; ++q) {
+                    if (var >= 0) push_slab_map(pmap[var][q], id, lb);
+                    else push_slab_map(lmap[lv][q], id, lb);
+                }
                 nlpass = std::max(nlpass, np);
             }
         }
@@ -753,7 +765,7 @@ static void iteration_bs(const Group<T> &G, const MapSet &M, hipStream_t s) {
     if (line) line_sweeps(G, M, 0, s);
     pf.begin(s);
     if constexpr (sizeof(T) == sizeof(float))
-        hipLaunchKernelGGL((k_matvec_dots2<BS, T, 0>), gs, dim3(BS / 2), 0, s, G.d_sys, M.d_smap);
+        hipLaunchKernelGGL((k_matvec_dots4<BS, 0>), gs, dim3(BS / 4), 0, s, (const Sys<float> *)G.d_sys, M.d_smap);
     else
         hipLaunchKernelGGL((k_matvec_dots<BS, T, 0>), gs, blk, 0, s, G.d_sys, M.d_smap);
     pf.end(s, M.k1_bytes);
@@ -761,7 +773,7 @@ static void iteration_bs(const Group<T> &G, const MapSet &M, hipStream_t s) {
     hipLaunchKernelGGL((k_s_update<BS, T>), gs, gblk, 0, s, G.d_sys, M.d_smap);
     if (line) line_sweeps(G, M, 1, s);
     if constexpr (sizeof(T) == sizeof(float))
-        hipLaunchKernelGGL((k_matvec_dots2<BS, T, 1>), gs, dim3(BS / 2), 0, s, G.d_sys, M.d_smap);
+        hipLaunchKernelGGL((k_matvec_dots4<BS, 1>), gs, dim3(BS / 4), 0, s, (const Sys<float> *)G.d_sys, M.d_smap);
     else
         hipLaunchKernelGGL((k_matvec_dots<BS, T, 1>), gs, blk, 0, s, G.d_sys, M.d_smap);
     finish<T, FIN_OMEGA, 3>(G, M, s);
@@ -1158,11 +1170,43 @@ int run_matvec64(const DGrid &g, const double *a, const double *x, double *y, hi
     return 0;
 }
 
+// The FP32 row of the inner solve through the solver's own first stencil
+// pass (k_matvec_dots4, operand x, result y; its dot partial goes to
+// scratch): a one-system table with a blank state, so the kernel never
+// returns early, and every logical block mapped in order.
 int run_matvec32(const DGrid &g, const float *a, const float *x, float *y, hipStream_t s) {
-    const Shape sh = ctx().shape_for(g.n3, true);
-    const Part P = make_part(g.n3, sh.bs, sh.cpt);
-    MFIX_DISPATCH_BS(sh.bs, BS, hipLaunchKernelGGL((k_matvec<BS, float>), grid_of(P), dim3(BS), 0, s, g, P, a, x, y));
-    return 0;
+    Context &C = ctx();
+    const Shape sh = C.shape_for(g.n3, true);
+    const Part P = make_part(g.n3, sh.bs, 1);
+    const size_t o_st = align256(sizeof(Sys<float>));
+    const size_t o_part = o_st + align256(sizeof(SolverState));
+    const size_t o_map = o_part + align256(sizeof(double) * kMaxPartials * (size_t)P.nblk);
+    int rc;
+    if ((rc = C.line1.ensure(o_map + sizeof(int2) * (size_t)P.nblk))) return rc;
+    char *base = (char *)C.line1.ptr;
+    Sys<float> S = {};
+    S.g = g;
+    S.P = P;
+    S.pc = PC_NONE;
+    S.a = a;
+    S.p = const_cast<float *>(x);
+    S.rt = const_cast<float *>(x);
+    S.v = y;
+    S.st = (SolverState *)(base + o_st);
+    S.partials = (double *)(base + o_part);
+    std::vector<int2> map((size_t)P.nblk);
+    for (int64_t b = 0; b < P.nblk; ++b) map[(size_t)b] = make_int2(0, (int)b);
+    hipError_t e = hipMemcpyAsync(base, &S, sizeof(S), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(base + o_map, map.data(), sizeof(int2) * map.size(), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return hip_fail(e, "matvec table upload");
+    hipLaunchKernelGGL(k_init_state, dim3(1), dim3(1), 0, s, S.st);
+    const Sys<float> *tab = (const Sys<float> *)base;
+    const int2 *mp = (const int2 *)(base + o_map);
+    MFIX_DISPATCH_BS(sh.bs, BS,
+                     hipLaunchKernelGGL((k_matvec_dots4<BS, 0>), dim3((unsigned)P.nblk), dim3(BS / 4), 0, s, tab, mp));
+    // the table must outlive the launch before it is rewritten
+    e = hipStreamSynchronize(s);
+    return e == hipSuccess ? 0 : hip_fail(e, "matvec32");
 }
 
 int run_diag(const DGrid &g, const double *a, const double *r, double *z, hipStream_t s) {

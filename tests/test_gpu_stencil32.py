@@ -1,6 +1,7 @@
-"""GPU numerics of the FP32 stencil row of the mixed-precision inner solve
-(`Stencil<float>::row`, shared by the solver's stencil passes and
-`mfix_leq_matvec32_dev`).
+"""GPU numerics of the FP32 stencil row of the mixed-precision inner solve:
+`mfix_leq_matvec32_dev` runs the solver's own first stencil pass
+(`k_matvec_dots4`: four cells per thread with 16-byte accesses where the
+grid allows, `Stencil<float>::row` per cell elsewhere).
 
 The FP32 matrix is in difference form: y(c) = sum_l a_l * (x_nb(l) - x_c)
 + e_c * x_c, where e_c = A0 + sum_l a_l is the row excess held in plane 3.
@@ -73,7 +74,19 @@ def _restated(a, x, nb, do_k):
     return v.astype(np.float32)
 
 
-@pytest.mark.parametrize("case", GRIDS, ids=[c[0] for c in GRIDS])
+# The solver's FP32 stencil pass takes four consecutive cells per thread on
+# 3-D grids without cyclic axes, off the first and last K planes: grids whose
+# J rows and N3 are not multiples of four (groups straddling rows and planes,
+# coefficient planes not 16-byte aligned), and one whose planes span several
+# waves, next to the per-cell cases above.
+STENCIL_GRIDS = GRIDS + [
+    ("v_33x29x17", 2, 33, 29, 17, False, (0, 0, 0)),
+    ("p_40x62x22", 0, 40, 62, 22, False, (0, 0, 0)),
+    ("m_64x64x64", 1, 64, 64, 64, False, (0, 0, 0)),
+]
+
+
+@pytest.mark.parametrize("case", STENCIL_GRIDS, ids=[c[0] for c in STENCIL_GRIDS])
 def test_fp32_stencil_row(gpu, oracle_mod, case):
     name, _kind, im, jm, km, no_k, cyc = case
     gpu.grid_init(im, jm, km, no_k=no_k, cyclic=tuple(bool(v) for v in cyc))
