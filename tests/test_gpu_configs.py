@@ -17,8 +17,9 @@
 * Config 4 (a serving stream drawn from the config-4 distribution): every
   request's result equals the same request served alone, bit for bit, and
   meets its tolerance on the FP64 true residual.
-* 66^3 (more than 2^18 cells: the multi-block finish and the mixed solver's
-  restart-at-every-update policy): FP64 DIAG against the oracle, and mixed.
+* 66^3 (more than 2^18 cells: the mixed solver's restart-at-every-update
+  policy): FP64 DIAG against the oracle, and mixed.  The several-block
+  finish (more than 16,384 partials) runs in the 256^3 solves of config 2.
 """
 from pathlib import Path
 
@@ -274,8 +275,8 @@ def test_config4_serving_stream(gpu, oracle_mod, mixed):
 
 
 def test_66cubed_fp64_diag_and_mixed(gpu, oracle_mod):
-    """287,496 cells (> 2^18): several finish blocks per reduction and the
-    mixed solver's restart-at-every-update policy."""
+    """287,496 cells (> 2^18): the mixed solver's restart-at-every-update
+    policy."""
     n = 66
     print("66^3 pressure: FP64 DIAG vs oracle, mixed true residual", flush=True)
     a, b = systems.gen_system(systems.VOID_PRESSURE, n, n, n, 66, nspheres=40)

@@ -174,18 +174,20 @@ def test_line_msolve_multi_tile_pivoting(gpu, oracle_mod):
 
 @pytest.mark.parametrize("serial", [True, False])
 def test_dot_product_multi_block_finish(gpu, oracle_mod, serial):
-    """66^3 (287,496 cells): more than 1024 partials per reduction, so the
-    finish runs in several blocks (level-2 sums, release increment, acquire
-    in the last block)."""
-    og, a, b = _setup(gpu, oracle_mod, 0, 66, 66, 66, False, (0, 0, 0), is_serial=serial)
-    x, y = rng_vec(b.size, 21), rng_vec(b.size, 22)
+    """170^3 (5,088,448 cells): more than 16,384 partials per reduction, so
+    the finish runs in several blocks (level-2 sums, release increment,
+    acquire in the last block); below that one block sums every partial."""
+    n = 170
+    gpu.grid_init(n, n, n, is_serial=serial)
+    nn = n + 2
+    x, y = rng_vec(nn ** 3, 21), rng_vec(nn ** 3, 22)
     dx, dy = gpu.DeviceArray.from_host(x), gpu.DeviceArray.from_host(y)
     for rep in range(3):  # the finish counter is reset by each launch's last block
         got = gpu.dot_dev(dx, dy)
         if serial:
             exact = math.fsum(x * y)
         else:
-            m = np.zeros((68, 68, 68), bool)
+            m = np.zeros((nn, nn, nn), bool)
             m[1:-1, 1:-1, 1:-1] = True
             exact = math.fsum((x * y)[m.reshape(-1)])
         assert abs(got - exact) <= 1e-14 * np.sum(np.abs(x * y)), (rep, got, exact)
