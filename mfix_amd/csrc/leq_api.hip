@@ -237,9 +237,12 @@ int mfix_leq_msolve32_dev(const char *cmethod, const float *d_a_m, const float *
         e = hipMemcpyAsync(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost, s);
         if (e != hipSuccess) return hip_fail(e, "copy");
         if ((rc = sync_status(s, "msolve32"))) return rc;
+        for (auto &pp : C.prof) pp.collect();
         return hflag ? MFIX_LEQ_EDGTSV : 0;
     }
-    return sync_status(s, "msolve32");
+    if ((rc = sync_status(s, "msolve32"))) return rc;
+    for (auto &pp : C.prof) pp.collect();
+    return 0;
 }
 
 int mfix_leq_matvec32_dev(const float *d_a_m, const float *d_x, float *d_y, void *stream) {

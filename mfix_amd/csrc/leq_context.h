@@ -53,6 +53,7 @@ struct Shape {
 };
 
 int size_class(int64_t n3);
+int default_block_size(int64_t n3, bool fp32);
 int default_lines_per_block(int64_t n3);
 double mixed_delta(int64_t n3);
 

@@ -963,10 +963,15 @@ __host__ __device__ inline int line_lpb_variant(int lpb) { return lpb >= 64 ? 0 
 //   3. each lane recovers its rows from G_l and G_{l-1}.
 // No pivoting: MFIX's rows are diagonally dominant, and a line whose result
 // is not finite keeps its values, as a failed DGTSV leaves it
-// (leqsol_mod.f:910-914).  The loads are the line's own rows, contiguous in
-// J, so a lane group reads whole line segments and nothing is staged through
-// LDS or scratch.  Rows past n are identity rows (x = 0, no coupling: the
-// last real row has no upper coupling).  Block b takes lines
+// (leqsol_mod.f:910-914).  Rows are loaded and stored in interleaved order
+// (lane gl takes rows gl, gl+P, ...: one load instruction reads P
+// consecutive rows of a line) and the right-hand side is formed there; the
+// four coefficients of the line's tridiagonal system go through LDS to the
+// chunk order the elimination needs, and the solution back the same way.
+// In LDS row q of chunk l = q / m, offset q - l*m, sits at l*s + offset with
+// a chunk stride s = m | 1 (odd, so the lanes' chunk reads fall in distinct
+// banks).  Rows past n are identity rows (x = 0, no coupling: the last real
+// row has no upper coupling).  Block b takes lines
 // [map[b].y*LPB, +LPB) of colour `param` (numbered densely by color_line), four
 // waves of 64/P lines.  which = 0 preconditions P into Phat, 1 S into Shat.
 template <int P, int M>
@@ -1002,15 +1007,26 @@ __global__ void __launch_bounds__(256) k_line_pcr_sys(const Sys<float> *__restri
     const gptr<const float> bm = to_global((const float *)(which == 0 ? S.p : S.s));
     const gptr<float> var = to_global(which == 0 ? S.phat : S.shat);
     const int do_k = g.do_k;
-    // rows q = gl*m + j of this lane: loads issued together, then the
-    // right-hand side in LEQ_IKSWEEP's order (leqsol_mod.f:891-902)
-    float ra[M], rb[M], rc[M], rd[M];
+    // LDS: per line, the four coefficient arrays of NS slots; lines LS apart
+    // (LS = P mod 64: the groups of a wave start in different banks)
+    constexpr int NS = P * (M + 1), LS = 4 * NS + P;
+    __shared__ float lds[LPB * LS];
+    float *__restrict__ ln = lds + ((threadIdx.x >> 6) * LPW + grp) * LS;
+    const int sc = m | 1;                 // chunk stride in LDS
+    const float rm = 1.f / (float)m;      // (q + 0.5) * rm truncates to q / m for q < 4096, m <= 8
+    auto slot = [&](int q) {
+        const int l = (int)(((float)q + 0.5f) * rm);
+        return l * sc + (q - l * m);
+    };
+    // rows q = gl + P*j: loads issued together, then the right-hand side in
+    // LEQ_IKSWEEP's order (leqsol_mod.f:891-902), and the line's tridiagonal
+    // coefficients to LDS
     {
-        float tb[M], t1m[M], t1p[M], v1m[M], v1p[M], t2m[M], t2p[M], v2m[M], v2p[M];
+        float tb[M], t1m[M], t1p[M], v1m[M], v1p[M], t2m[M], t2p[M], v2m[M], v2p[M], ta[M], tbd[M], tc[M];
 #pragma unroll
         for (int j = 0; j < M; ++j) {
-            const int q = gl * m + j;
-            const bool in = ok && j < m && q < n;
+            const int q = gl + P * j;
+            const bool in = ok && q < n;
             const uint32_t c = in ? c0 + (uint32_t)q : c0;  // in bounds even when not used
             tb[j] = in ? bm[c] : 0.f;
             t1m[j] = in ? pI_m[c] : 0.f;
@@ -1021,14 +1037,36 @@ __global__ void __launch_bounds__(256) k_line_pcr_sys(const Sys<float> *__restri
             t2p[j] = in && do_k ? pK_p[c] : 0.f;
             v2m[j] = in && do_k ? var[c + okm] : 0.f;
             v2p[j] = in && do_k ? var[c + okp] : 0.f;
-            rb[j] = in ? pD[c] : 1.f;
-            ra[j] = in && q > 0 ? pJ_m[c] : 0.f;
-            rc[j] = in && q < n - 1 ? pJ_p[c] : 0.f;
+            tbd[j] = in ? pD[c] : 1.f;
+            ta[j] = in && q > 0 ? pJ_m[c] : 0.f;
+            tc[j] = in && q < n - 1 ? pJ_p[c] : 0.f;
         }
 #pragma unroll
-        for (int j = 0; j < M; ++j)
-            rd[j] = do_k ? tb[j] - t1m[j] * v1m[j] - t1p[j] * v1p[j] - t2m[j] * v2m[j] - t2p[j] * v2p[j]
-                         : tb[j] - t1m[j] * v1m[j] - t1p[j] * v1p[j];
+        for (int j = 0; j < M; ++j) {
+            const int q = gl + P * j;
+            if (ok && q < n) {
+                const float d = do_k ? tb[j] - t1m[j] * v1m[j] - t1p[j] * v1p[j] - t2m[j] * v2m[j] - t2p[j] * v2p[j]
+                                     : tb[j] - t1m[j] * v1m[j] - t1p[j] * v1p[j];
+                const int o = slot(q);
+                ln[o] = ta[j];
+                ln[NS + o] = tbd[j];
+                ln[2 * NS + o] = tc[j];
+                ln[3 * NS + o] = d;
+            }
+        }
+    }
+    __syncthreads();
+    // this lane's chunk: rows q = gl*m + j
+    float ra[M], rb[M], rc[M], rd[M];
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+        const int q = gl * m + j;
+        const bool in = ok && j < m && q < n;
+        const int o = gl * sc + j;
+        ra[j] = in ? ln[o] : 0.f;
+        rb[j] = in ? ln[NS + o] : 1.f;
+        rc[j] = in ? ln[2 * NS + o] : 0.f;
+        rd[j] = in ? ln[3 * NS + o] : 0.f;
     }
     // 1. rows 0 .. m-2 of the chunk: forward elimination carrying the
     // coupling to the previous lane's separator, then backward substitution
@@ -1114,22 +1152,28 @@ __global__ void __launch_bounds__(256) k_line_pcr_sys(const Sys<float> *__restri
     const float G = D / B;
     float Gp = __shfl(G, base + (gl > 0 ? gl - 1 : 0), 64);
     if (gl == 0) Gp = 0.f;
-    // 3. the chunk's rows; a line with a non-finite value keeps its values
-    float x[M];
+    // 3. the chunk's rows, back to LDS (this lane's own slots of the
+    // right-hand side array, read above by this lane only); a line with a
+    // non-finite value keeps its values
     int bad = 0;
 #pragma unroll
     for (int j = 0; j < M; ++j) {
-        x[j] = j < m - 1 ? U[j] - V[j] * G - W[j] * Gp : G;
+        const float x = j < m - 1 ? U[j] - V[j] * G - W[j] * Gp : G;
         const int q = gl * m + j;
-        if (ok && j < m && q < n) bad |= !isfinite(x[j]);
+        if (ok && j < m && q < n) {
+            bad |= !isfinite(x);
+            ln[3 * NS + gl * sc + j] = x;
+        }
     }
     const uint64_t bm_bad = __ballot(bad);
     const uint64_t gmask = P == 64 ? ~0ull : (((1ull << P) - 1ull) << base);
-    if (!ok || (bm_bad & gmask)) return;
+    const bool store = ok && !(bm_bad & gmask);
+    __syncthreads();
+    if (!store) return;
 #pragma unroll
     for (int j = 0; j < M; ++j) {
-        const int q = gl * m + j;
-        if (j < m && q < n) var[c0 + (uint32_t)q] = x[j];
+        const int q = gl + P * j;
+        if (q < n) var[c0 + (uint32_t)q] = ln[3 * NS + slot(q)];
     }
 }
 
@@ -1138,16 +1182,32 @@ __global__ void __launch_bounds__(256) k_line_pcr_sys(const Sys<float> *__restri
 __host__ __device__ inline int line_pcr_variant(int n) {
     if (n <= 32) return 0;   // P 16, M 2
     if (n <= 64) return 1;   // P 16, M 4
-    if (n <= 128) return 2;  // P 16, M 8
-    if (n <= 256) return 3;  // P 32, M 8
-    if (n <= 320) return 4;  // P 64, M 5
-    if (n <= 512) return 5;  // P 64, M 8
+    if (n <= 128) return 2;  // P 32, M 4
+    if (n <= 160) return 3;  // P 32, M 5
+    if (n <= 256) return 4;  // P 64, M 4
+    if (n <= 320) return 5;  // P 64, M 5
+    if (n <= 512) return 6;  // P 64, M 8
     return -1;
 }
-constexpr int kPcrVariants = 6;
-__host__ __device__ inline int line_pcr_lanes(int var) { return var <= 2 ? 16 : var == 3 ? 32 : 64; }
+constexpr int kPcrVariants = 7;
+__host__ __device__ inline int line_pcr_lanes(int var) { return var <= 1 ? 16 : var <= 3 ? 32 : 64; }
 // lines per block of a variant (four waves of 64/P lines)
 __host__ __device__ inline int line_pcr_lpb(int var) { return 4 * (64 / line_pcr_lanes(var)); }
+
+// Launch the k_line_pcr_sys variant `var` (line_pcr_variant) over `grid` blocks.
+inline void launch_line_pcr(int var, dim3 grid, hipStream_t s, const Sys<float> *tab, const int2 *map,
+                            int param, int which) {
+    const dim3 bs(256);
+    switch (var) {
+    case 0: hipLaunchKernelGGL((k_line_pcr_sys<16, 2>), grid, bs, 0, s, tab, map, param, which); break;
+    case 1: hipLaunchKernelGGL((k_line_pcr_sys<16, 4>), grid, bs, 0, s, tab, map, param, which); break;
+    case 2: hipLaunchKernelGGL((k_line_pcr_sys<32, 4>), grid, bs, 0, s, tab, map, param, which); break;
+    case 3: hipLaunchKernelGGL((k_line_pcr_sys<32, 5>), grid, bs, 0, s, tab, map, param, which); break;
+    case 4: hipLaunchKernelGGL((k_line_pcr_sys<64, 4>), grid, bs, 0, s, tab, map, param, which); break;
+    case 5: hipLaunchKernelGGL((k_line_pcr_sys<64, 5>), grid, bs, 0, s, tab, map, param, which); break;
+    default: hipLaunchKernelGGL((k_line_pcr_sys<64, 8>), grid, bs, 0, s, tab, map, param, which); break;
+    }
+}
 
 // Number of the listed systems still running, and a running flag per
 // listed system.

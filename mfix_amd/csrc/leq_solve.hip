@@ -182,15 +182,23 @@ void Prof::collect(int64_t working) {
 
 // Shape of the streaming kernels for a grid of n3 cells.  The autotuner
 // (mfix_leq_autotune) times the candidates on the device and records the
-// fastest per size class and precision; untuned classes use one cell per
-// thread in 256-thread blocks, the shape that streamed best when measured.
+// fastest per size class and precision; untuned classes use
+// default_block_size.
 Shape Context::shape_for(int64_t n3, bool fp32) const {
     const int cls = size_class(n3);
     Shape s = tuned[fp32 ? 1 : 0][cls];
-    if (!s.bs) s = Shape{256, 1, 0};
+    if (!s.bs) s = Shape{default_block_size(n3, fp32), 1, 0};
     if (!s.lines_per_block) s.lines_per_block = default_lines_per_block(n3);
     return s;
 }
+
+// Cells per logical block of the stencil passes when none was tuned: 256,
+// and 512 for FP32 systems of more than 2^23 cells.  Measured on the 256^3
+// mixed DIAG solve (two solves each): 476.0-478.8 us per iteration with 256
+// cells, 471.1-471.2 with 512, 471.8-472.1 with 1024 (the finish of 512- and
+// 1024-cell blocks sums 33.5K / 16.8K partials in two levels; the single-block
+// finish over 16.8K partials ran 474.6-474.9).
+int default_block_size(int64_t n3, bool fp32) { return fp32 && n3 > (int64_t(1) << 23) ? 512 : 256; }
 
 // Lines per block of the red-black line kernel when none was tuned: 64.
 // A colour phase is bound by the chain of dependent memory round trips of
@@ -411,15 +419,8 @@ int run_msolve_line_rb(const DGrid &g, const char *cmethod, const T *a, const T 
             pf.begin(s);
             if constexpr (sizeof(T) == sizeof(float)) {
                 const Sys<float> *tf = (const Sys<float> *)tab;
-                switch (var) {
-                case 0: hipLaunchKernelGGL((k_line_pcr_sys<16, 2>), gs, dim3(256), 0, s, tf, mp, param, 0); break;
-                case 1: hipLaunchKernelGGL((k_line_pcr_sys<16, 4>), gs, dim3(256), 0, s, tf, mp, param, 0); break;
-                case 2: hipLaunchKernelGGL((k_line_pcr_sys<16, 8>), gs, dim3(256), 0, s, tf, mp, param, 0); break;
-                case 3: hipLaunchKernelGGL((k_line_pcr_sys<32, 8>), gs, dim3(256), 0, s, tf, mp, param, 0); break;
-                case 4: hipLaunchKernelGGL((k_line_pcr_sys<64, 5>), gs, dim3(256), 0, s, tf, mp, param, 0); break;
-                case 5: hipLaunchKernelGGL((k_line_pcr_sys<64, 8>), gs, dim3(256), 0, s, tf, mp, param, 0); break;
-                default: launch_line_color<float>(lv, gs, s, tf, mp, param); break;
-                }
+                if (var >= 0) launch_line_pcr(var, gs, s, tf, mp, param, 0);
+                else launch_line_color<float>(lv, gs, s, tf, mp, param);
             } else {
                 launch_line_color<T>(lv, gs, s, tab, mp, param);
             }
@@ -738,17 +739,9 @@ static void line_sweeps(const Group<T> &G, const MapSet &M, int which, hipStream
             if constexpr (sizeof(T) == sizeof(float)) {
                 for (int v = 0; v < kPcrVariants; ++v) {
                     if (M.pmap[v][q].empty()) continue;
-                    const dim3 gs((unsigned)M.pmap[v][q].size()), bs(256);
                     const Sys<float> *tab = (const Sys<float> *)G.d_sys;
                     pf.begin(s);
-                    switch (v) {
-                    case 0: hipLaunchKernelGGL((k_line_pcr_sys<16, 2>), gs, bs, 0, s, tab, M.d_pmap[v][q], param, which); break;
-                    case 1: hipLaunchKernelGGL((k_line_pcr_sys<16, 4>), gs, bs, 0, s, tab, M.d_pmap[v][q], param, which); break;
-                    case 2: hipLaunchKernelGGL((k_line_pcr_sys<16, 8>), gs, bs, 0, s, tab, M.d_pmap[v][q], param, which); break;
-                    case 3: hipLaunchKernelGGL((k_line_pcr_sys<32, 8>), gs, bs, 0, s, tab, M.d_pmap[v][q], param, which); break;
-                    case 4: hipLaunchKernelGGL((k_line_pcr_sys<64, 5>), gs, bs, 0, s, tab, M.d_pmap[v][q], param, which); break;
-                    default: hipLaunchKernelGGL((k_line_pcr_sys<64, 8>), gs, bs, 0, s, tab, M.d_pmap[v][q], param, which); break;
-                    }
+                    launch_line_pcr(v, dim3((unsigned)M.pmap[v][q].size()), s, tab, M.d_pmap[v][q], param, which);
                     pf.end(s, 0.5 * M.cells * 13.0 * sizeof(T));
                 }
             }
@@ -1351,7 +1344,9 @@ int autotune(int imax, int jmax, int kmax, int mixed, Shape *best_out) {
     };
     // threads per block (one logical block = one block of the stencil
     // passes; the grouped update kernels cover it with 16-byte cell groups)
-    const int bss[] = {256, 512, 1024};
+    // (the size default first: a candidate replaces it only when faster)
+    const int db0 = default_block_size(n3, mixed);
+    const int bss[] = {db0, db0 == 256 ? 512 : 256, 1024};
     const int bi = pick(PC_DIAG, dp, dbp, 3, [&](int c) { slot = Shape{bss[c], 1, 0}; });
     Shape best = Shape{bss[bi], 1, 0};
     // lines per block of the red-black line kernel: the size default first

@@ -24,8 +24,11 @@ LINE32 = [
     ("j30", 0, 5, 30, 6, False, (0, 0, 0)),      # 32 rows: lanes exactly full
     ("j50", 1, 6, 50, 5, False, (0, 0, 1)),
     ("j100", 1, 5, 100, 4, False, (0, 0, 0)),
+    ("j126", 0, 4, 126, 3, False, (0, 0, 0)),    # 128 rows: 32 lanes of 4, full
     ("j128", 1, 4, 128, 4, False, (0, 0, 0)),    # the 128^3 line length
+    ("j158", 1, 3, 158, 3, False, (1, 0, 0)),    # 160 rows: 32 lanes of 5, full
     ("j200", 0, 4, 200, 3, False, (0, 0, 0)),
+    ("j254", 1, 3, 254, 3, False, (0, 0, 0)),    # 256 rows: 64 lanes of 4, full
     ("j256", 1, 3, 256, 3, False, (0, 0, 0)),    # the 256^3 line length
     ("j300", 1, 3, 300, 3, False, (0, 0, 0)),
     ("j450", 1, 3, 450, 3, False, (0, 0, 0)),

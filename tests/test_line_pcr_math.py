@@ -8,13 +8,16 @@ recovers its rows (partition method, Wang 1981; PCR, Hockney 1965).  This
 test restates those steps in numpy, lane by lane as the kernel orders them,
 and checks them in FP64 against scipy's banded LU on diagonally dominant
 lines of every length class the kernel's variants take, including lines
-shorter than P (idle lanes) and lengths that fill the lanes exactly.
+shorter than P (idle lanes) and lengths that fill the lanes exactly.  It
+also checks the kernel's LDS layout: rows are loaded in interleaved order
+(row q by lane q mod P) and read back by chunk (rows gl*m .. gl*m+m-1 by
+lane gl), through slots l*s + (q - l*m) with l = q // m and s = m | 1.
 """
 import numpy as np
 import pytest
 from scipy.linalg import solve_banded
 
-VARIANTS = [(16, 2), (16, 4), (16, 8), (32, 8), (64, 5), (64, 8)]
+VARIANTS = [(16, 2), (16, 4), (32, 4), (32, 5), (64, 4), (64, 5), (64, 8)]
 
 
 def partition_pcr(a, b, c, d, P, M):
@@ -106,3 +109,45 @@ def test_planted_sign_error_is_caught():
     x_bad = partition_pcr(a, b, -c, d, 16, 8)
     y_bad = b * x_bad + np.append(0.0, a[1:] * x_bad[:-1]) + np.append(c[:-1] * x_bad[1:], 0.0)
     assert np.max(np.abs(y_bad - d)) > 1e-3
+
+
+def _slot_f32(q, m):
+    """The kernel's slot of row q: q // m from one FP32 multiply by 1/m."""
+    rm = np.float32(1.0) / np.float32(m)
+    l = int(np.float32(np.float32(q) + np.float32(0.5)) * rm)
+    return l * (m | 1) + (q - l * m), l
+
+
+@pytest.mark.parametrize("P,M", VARIANTS)
+def test_lds_slots_transpose_rows_to_chunks(P, M):
+    """Every row of every length the variant takes lands in its own slot
+    below P*(M+1), a lane's chunk reads find exactly its rows, and the
+    chunk reads of the P lanes fall in P distinct banks of 64."""
+    for n in range(1, P * M + 1):
+        m = (n + P - 1) // P
+        if m > M:
+            continue
+        slots = {}
+        for q in range(n):
+            o, l = _slot_f32(q, m)
+            assert l == q // m, (P, M, n, q)
+            assert 0 <= o < P * (M + 1)
+            slots[o] = q
+        assert len(slots) == n
+        for gl in range(P):
+            for j in range(m):
+                q = gl * m + j
+                if q < n:
+                    assert slots[gl * (m | 1) + j] == q
+        for j in range(m):
+            banks = {(gl * (m | 1) + j) % 64 for gl in range(P)}
+            assert len(banks) == P
+
+
+def test_slot_division_exact_for_every_row_index():
+    """(q + 0.5) * fl(1/m) in FP32 truncates to q // m for q < 4096, m <= 8
+    (the kernel's lines have at most 512 rows)."""
+    for m in range(1, 9):
+        q = np.arange(4096, dtype=np.float32)
+        l = ((q + np.float32(0.5)) * (np.float32(1.0) / np.float32(m))).astype(np.int64)
+        assert np.array_equal(l, np.arange(4096) // m), m
