@@ -1274,8 +1274,9 @@ int run_copy(const double *src, double *dst, int64_t n, hipStream_t s) {
 
 // ---- autotuner ------------------------------------------------------------------
 
-// Time a few iterations of the solve for each candidate on a seeded system of
-// the given size and keep the fastest per (size class, precision): the
+// Time 40 iterations of a DIAG solve (8 of a LINE solve) for each candidate
+// on a seeded system of the given size and keep the fastest per (size class,
+// precision): the
 // threads per block of the streaming kernels on a DIAG solve of a
 // void-fraction pressure system, then the lines per block of the red-black
 // line kernel on a LINE solve of an upwind momentum system.  Candidates are
@@ -1318,7 +1319,7 @@ int autotune(int imax, int jmax, int kmax, int mixed, Shape *best_out) {
     // median time per iteration of each candidate, timed round-robin
     auto pick = [&](int pc, const double *a0, const double *b0, int ncand, auto set_cand) -> int {
         constexpr int kRounds = 5;
-        const int iters = pc == PC_LINE ? 4 : 12;
+        const int iters = pc == PC_LINE ? 8 : 40;  // set-up under 10 % of a DIAG candidate at 256^3
         std::vector<std::vector<double>> t((size_t)ncand);
         for (int round = 0; round < kRounds && !rc; ++round)
             for (int c = 0; c < ncand && !rc; ++c) {
