@@ -804,9 +804,26 @@ static void enqueue_count(const Group<T> &G, const MapSet &M, int *d_out, int *d
                        d_flags);
 }
 
+// Line-kernel launch of a LINE system: 1 + its variant index among the
+// red-black kernels (the FP32 several-lanes-per-line variants, or the FP64
+// lines-per-block variants), 0 for other systems and one-by-one sweeps.
+template <typename T>
+static int line_launch_of(const Group<T> &G, int id) {
+    const Sys<T> &S = G.sys[id];
+    if (S.pc != PC_LINE || rb_passes(G.sweep[id], S.g.do_k != 0) < 0) return 0;
+    if (sizeof(T) == sizeof(float)) {
+        const int var = line_pcr_variant(line_geom(S.g, AX_J).n);
+        if (var >= 0) return 1 + kLineLpbVariants + var;
+    }
+    return 1 + line_lpb_variant(ctx().shape_for(S.g.n3, sizeof(T) == sizeof(float)).lines_per_block);
+}
+
 // Drop finished systems from a table's block maps once the running systems
-// hold less than 60 % of the mapped cells, so later launches stop
-// dispatching their blocks (a handful of rebuilds per batch).
+// hold less than 60 % of the mapped cells, or once every system of one of
+// the line-kernel launches is done (each colour phase launches every line
+// kernel variant that holds a system, so a variant of finished systems would
+// dispatch a launch of idle blocks 8 times per iteration): later launches
+// stop dispatching their blocks (a handful of rebuilds per batch).
 template <typename T>
 static int compact(Group<T> &G, const int *d_flags, int running, hipStream_t s) {
     MapSet &M = G.iter;
@@ -817,12 +834,20 @@ static int compact(Group<T> &G, const int *d_flags, int running, hipStream_t s) 
     if (e != hipSuccess) return hip_fail(e, "running flags");
     std::vector<int> keep;
     double cells = 0.0;
-    for (size_t q = 0; q < flags.size(); ++q)
+    constexpr int kLaunches = 1 + kLineLpbVariants + kPcrVariants;
+    bool mapped[kLaunches] = {}, running_in[kLaunches] = {};
+    for (size_t q = 0; q < flags.size(); ++q) {
+        const int l = line_launch_of(G, M.ids[q]);
+        mapped[l] = true;
         if (flags[q]) {
             keep.push_back(M.ids[q]);
             cells += (double)G.sys[M.ids[q]].g.n3;
+            running_in[l] = true;
         }
-    if (cells >= 0.6 * M.cells) return 0;
+    }
+    bool idle_launch = false;
+    for (int l = 1; l < kLaunches; ++l) idle_launch |= mapped[l] && !running_in[l];
+    if (cells >= 0.6 * M.cells && !idle_launch) return 0;
     return build_maps<T>(G, keep, M, s);
 }
 
