@@ -685,10 +685,9 @@ __global__ void __launch_bounds__(kFinishBlock) k_finish_sys(const Sys<T> *__res
 // the I/K neighbour terms, whose offsets are constant along a J-line) and
 // stages it in LDS, while wave 0 eliminates the previous tile: one memory
 // round trip per tile (16 rows at 64 lines per block, 32 at 32, 64 at 16).
-// Finished rows go to scratch in [row][lane] order; the back solve reads
-// them in groups of 8 rows, the next group's loads in flight while the
-// current one is solved, and waves 1-2 write the solution out through LDS,
-// coalesced.  Fewer lines per block give more blocks (128^3: 8,450 lines per
+// Finished rows go to scratch in [row][lane] order; for the back solve
+// waves 1-4 stage them in LDS in groups of 8 rows, two groups ahead, and
+// waves 1-2 write the solution out through LDS, coalesced.  Fewer lines per block give more blocks (128^3: 8,450 lines per
 // colour are 133 blocks of 64 but 265 of 32); the launch shape picks LPB
 // per grid size.  Four loaders of four elements each keep the kernel at two
 // blocks per CU (a loader with eight elements needs 256 registers, one
@@ -877,36 +876,45 @@ __global__ void __launch_bounds__(320) k_line_color_sys(const Sys<T> *__restrict
         sjc[k] = __shfl(vld, src, 64) && !linfo[src] ? (uint32_t)su * c1 + (uint32_t)sv * c2 : ~0u;
     }
     // back solve (DGTSV.f:233-246) in groups of RB rows from the top row
-    // down; rows n-2 .. 0 come from scratch (row n-1 is still in registers)
-    T x1 = mine ? b0 / d0 : (T)0, x2 = (T)0;
-    T gD[RB], gU[RB], gU2[RB], gB[RB];
-    auto fetch = [&](int hi) __attribute__((always_inline)) {
-        const int lo = hi - RB + 1 > 0 ? hi - RB + 1 : 0;
+    // down; rows n-2 .. 0 come from scratch (row n-1 is still in registers).
+    // Waves 1-4 stage the scratch rows of group t+1 in LDS (the tiles'
+    // buffers, free now) while wave 0 solves group t, with the loads of the
+    // next DB groups in flight in their registers: the back solve waits on
+    // memory once, not once per group.
+    constexpr int GE = 4 * RB * LPB;  // scratch values of one group
+    constexpr int FE = GE / 256;      // per lane of the four staging waves
+    constexpr int DB = 2;             // groups in flight per staging lane
+    static_assert(2 * GE <= 2 * 4 * RT * SL, "two staged groups fit in the tile buffers");
+    T(*bs)[4][RB][LPB] = reinterpret_cast<T(*)[4][RB][LPB]>(&tl[0][0][0][0]);  // slot t & 1
+    const int ngrp = (n + RB - 1) / RB;
+    const bool stager = wave >= 1;
+    const int fl = (wave - 1) * 64 + lane;  // 0..255 on the staging waves
+    T fr[DB][FE];
+    auto issue = [&](int t, T(&dst)[FE]) __attribute__((always_inline)) {
+        const int hi = n - 1 - t * RB, lo = hi - RB + 1 > 0 ? hi - RB + 1 : 0;
 #pragma unroll
-        for (int r = 0; r < RB; ++r) {
+        for (int k = 0; k < FE; ++k) {
+            const int e = fl + 256 * k, vq = e / (RB * LPB), r = (e / LPB) % RB, ln = e % LPB;
             const int q = hi - r;
-            if (q >= lo && q <= n - 2) {
-                const uint32_t o = (uint32_t)q * LPB + lane;
-                gD[r] = sc[o];
-                gU[r] = sc[pl + o];
-                gU2[r] = sc[2 * pl + o];
-                gB[r] = sc[3 * pl + o];
-            }
+            const bool ok = t < ngrp && q >= lo && q <= n - 2;
+            dst[k] = ok ? sc[(uint32_t)vq * pl + (uint32_t)q * LPB + ln] : (T)0;
         }
     };
-    if (wave == 0 && mine) fetch(n - 1);
-    for (int hi = n - 1; hi >= 0; hi -= RB) {
-        const int lo = hi - RB + 1 > 0 ? hi - RB + 1 : 0;
-        if (wave == 0 && mine) {
-            T cD[RB], cU[RB], cU2[RB], cB[RB];
+    auto stage = [&](int t, const T(&src)[FE]) __attribute__((always_inline)) {
 #pragma unroll
-            for (int r = 0; r < RB; ++r) {
-                cD[r] = gD[r];
-                cU[r] = gU[r];
-                cU2[r] = gU2[r];
-                cB[r] = gB[r];
-            }
-            if (lo > 0) fetch(lo - 1);  // next group in flight while this one is solved
+        for (int k = 0; k < FE; ++k) {
+            const int e = fl + 256 * k, vq = e / (RB * LPB), r = (e / LPB) % RB, ln = e % LPB;
+            bs[t & 1][vq][r][ln] = src[k];
+        }
+    };
+    T x1 = mine ? b0 / d0 : (T)0, x2 = (T)0;
+    // group t: solve it from LDS (wave 0), stage group t+1 and put group
+    // t+1+DB in flight in the register set group t+1 held (waves 1-4; set
+    // (t+1) % DB, so the loop is unrolled by DB), then store its solution
+    // rows (waves 1-2)
+    auto group = [&](int t, T(&set)[FE]) __attribute__((always_inline)) {
+        const int hi = n - 1 - t * RB, lo = hi - RB + 1 > 0 ? hi - RB + 1 : 0;
+        if (wave == 0 && mine) {
 #pragma unroll
             for (int r = 0; r < RB; ++r) {
                 const int q = hi - r;
@@ -915,17 +923,21 @@ __global__ void __launch_bounds__(320) k_line_color_sys(const Sys<T> *__restrict
                     if (q == n - 1) {
                         x = x1;
                     } else if (q == n - 2) {
-                        x = (cB[r] - cU[r] * x1) / cD[r];
+                        x = (bs[t & 1][3][r][lane] - bs[t & 1][1][r][lane] * x1) / bs[t & 1][0][r][lane];
                         x2 = x1;
                         x1 = x;
                     } else {
-                        x = (cB[r] - cU[r] * x1 - cU2[r] * x2) / cD[r];
+                        x = (bs[t & 1][3][r][lane] - bs[t & 1][1][r][lane] * x1 - bs[t & 1][2][r][lane] * x2) /
+                            bs[t & 1][0][r][lane];
                         x2 = x1;
                         x1 = x;
                     }
                     xs[q - lo][lane] = x;
                 }
             }
+        } else if (stager && t + 1 < ngrp) {
+            stage(t + 1, set);
+            issue(t + 1 + DB, set);
         }
         __syncthreads();
         if (wave == 1 || wave == 2) {
@@ -937,6 +949,17 @@ __global__ void __launch_bounds__(320) k_line_color_sys(const Sys<T> *__restrict
             }
         }
         __syncthreads();
+    };
+    if (stager) {
+        issue(0, fr[0]);
+        issue(1, fr[1]);
+        stage(0, fr[0]);
+        issue(DB, fr[0]);
+    }
+    __syncthreads();
+    for (int t0 = 0; t0 < ngrp; t0 += DB) {
+        group(t0, fr[1]);
+        if (t0 + 1 < ngrp) group(t0 + 1, fr[0]);
     }
 }
 
