@@ -389,6 +389,24 @@ def bench_phi(args):
         r, out = chain()
         times.append(time.perf_counter() - t)
     dev_ms = 1e3 * sum(times) / len(times)
+
+    # the same chain with CALC_RESID_S and UNDER_RELAX_S done by the solver's
+    # set-up pass (mfix_leq_resid_relax_bicgs_dev)
+    def fused_chain():
+        leq.conv_dif_phi_dev(fields, flag, da, db, sync=False)
+        leq.source_phi_dev(*src, flag, da, db, sync=False)
+        return leq.resid_relax_bicgs_dev("S", fields["phi"], da, db, flag, 0.8, debug=False, **kw)
+    fused_times = []
+    for _ in range(args.steps + args.warmup):
+        fields["phi"].upload(phi0)
+        _lib.check(L.mfix_leq_sync(), "sync")
+        t = time.perf_counter()
+        fr, fout = fused_chain()
+        fused_times.append(time.perf_counter() - t)
+    fused_times = fused_times[args.warmup:]
+    fused_ms = 1e3 * sum(fused_times) / len(fused_times)
+    if fout.stats["iterations"] != out.stats["iterations"] or fr["ncells"] != r["ncells"]:
+        raise RuntimeError(f"fused chain differs: {fout.stats} {fr} against {out.stats} {r}")
     # host path: the assembled, under-relaxed system as a CPU assembly would hand it over
     fields["phi"].upload(phi0)
     leq.conv_dif_phi_dev(fields, flag, da, db)
@@ -414,6 +432,7 @@ def bench_phi(args):
                                "LINE RSRS, TOL 1e-4, ITMAX 15 (MFIX defaults, equation 9)",
                    "iterations": out.stats["iterations"], "ier": out.ier, "resid_num": r["num"],
                    "resid_den": r["den"], "kernels": per, "host_path_ms": host_ms, "host_path_ier": ier_h,
+                   "fused_resid_relax_chain_ms": fused_ms,
                    "host_copy_bytes_avoided": 10 * 8 * n3},
         "roofline": {"bound": "hbm", "kernel": "k_conv_dif_phi", "achieved": conv["alg_GBps"],
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": conv["frac"], "traffic": None},

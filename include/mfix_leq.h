@@ -147,6 +147,32 @@ int mfix_leq_bicgs_mixed_dev(const char *cmethod, double tol, const char *pc,
                              double *d_b_m, const mfix_leq_opts *opts,
                              mfix_leq_stats *stats, void *stream);
 
+struct mfix_leq_resid; /* defined with the residual entries below */
+
+/* CALC_RESID, UNDER_RELAX_S and LEQ_BICGS of one equation, the residual and
+ * the under-relaxation done by the solver's set-up pass (the pass that
+ * scales the rows and forms R = B - A*VAR) instead of two passes of their
+ * own over A_M, B_M and VAR: MFIX's sequence CALC_RESID_S, UNDER_RELAX_S,
+ * SOLVE_LIN_EQ (model/solve_scalar_eq.f:129-152,
+ * model/solve_energy_eq.f:253-279, model/solve_species_eq.f:163-175) or
+ * CALC_RESID_PP, SOLVE_LIN_EQ (model/solve_pp_g.f:104-119).  resid_kind: 0 =
+ * CALC_RESID_C, 1 = CALC_RESID_S (resid_tol, d_ep_s, dil_ep_s as in
+ * mfix_calc_resid_s_dev), 2 = CALC_RESID_PP (norm), -1 = none; the residual
+ * is of the equation as passed, before the under-relaxation.  ur: UR_FAC of
+ * UNDER_RELAX_S, 0 = none.  mixed: 0 = mfix_leq_bicgs_dev's FP64 solve, 1 =
+ * mfix_leq_bicgs_mixed_dev's.  Results: *resid as the mfix_calc_resid_*_dev
+ * call gives it (NUM and DEN to summation-order rounding, the rest
+ * exactly); VAR, A_M, B_M, the return code and *stats exactly as
+ * mfix_under_relax_s_dev followed by the solve.  No counterpart in the
+ * reference, which runs the routines in turn; device pointers. */
+int mfix_leq_resid_relax_bicgs_dev(int resid_kind, const int *d_flag, double resid_tol,
+                                   const double *d_ep_s, double dil_ep_s, double norm,
+                                   int debug_resid, struct mfix_leq_resid *resid, double ur,
+                                   const char *cmethod, double tol, const char *pc, int itmax,
+                                   int mixed, double *d_var, double *d_a_m, double *d_b_m,
+                                   const mfix_leq_opts *opts, mfix_leq_stats *stats,
+                                   void *stream);
+
 /* ---- kernels of the path, device pointers (for parity tests) ------------ */
 /* LEQ_MATVEC leqsol_mod.f:103-265: y = A x over every cell. */
 int mfix_leq_matvec_dev(const double *d_a_m, const double *d_x, double *d_y,

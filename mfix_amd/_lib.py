@@ -157,6 +157,10 @@ _SIGS = {
                                         C.c_double, C.c_int, C.c_void_p, C.c_void_p]),
     "mfix_calc_resid_pp": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.c_void_p]),
     "mfix_calc_resid_pp_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.c_void_p, C.c_void_p]),
+    "mfix_leq_resid_relax_bicgs_dev": (C.c_int, [C.c_int, C.c_void_p, C.c_double, C.c_void_p, C.c_double,
+                                                 C.c_double, C.c_int, C.c_void_p, C.c_double, C.c_char_p,
+                                                 C.c_double, C.c_char_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                                 C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "mfix_leq_dev_alloc": (C.c_int, [C.c_size_t, C.POINTER(C.c_void_p)]),
     "mfix_leq_dev_free": (C.c_int, [C.c_void_p]),
     "mfix_leq_copy_to_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),

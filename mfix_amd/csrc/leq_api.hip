@@ -3,6 +3,8 @@
 // engine (leq_solve.hip); none of them computes on the host.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+
 #include <cstring>
 #include <vector>
 
@@ -139,6 +141,36 @@ int mfix_leq_bicgs_mixed_dev(const char *cmethod, double tol, const char *pc, in
     if (!d_var || !d_a_m || !d_b_m) return MFIX_LEQ_EINVAL;
     return solve_mixed(make_dgrid(ctx().grid), p, cmethod ? cmethod : "RSRS", tol, itmax, d_var, d_a_m, d_b_m, opts,
                        stats, s);
+}
+
+int mfix_leq_resid_relax_bicgs_dev(int resid_kind, const int *d_flag, double resid_tol, const double *d_ep_s,
+                                   double dil_ep_s, double norm, int debug_resid, mfix_leq_resid *resid, double ur,
+                                   const char *cmethod, double tol, const char *pc, int itmax, int mixed,
+                                   double *d_var, double *d_a_m, double *d_b_m, const mfix_leq_opts *opts,
+                                   mfix_leq_stats *stats, void *stream) {
+    hipStream_t s;
+    int rc;
+    const int p = parse_pc(pc);
+    if (p < 0) return MFIX_LEQ_EBADPC;
+    if ((rc = ready(&s, stream))) return rc;
+    if (!d_var || !d_a_m || !d_b_m || !d_flag) return MFIX_LEQ_EINVAL;
+    if (resid_kind < -1 || resid_kind > RESID_PP || (resid_kind >= 0 && !resid)) return MFIX_LEQ_EINVAL;
+    if (!std::isfinite(ur)) return MFIX_LEQ_EINVAL;
+    Prologue pro;
+    pro.flag = d_flag;
+    pro.resid_kind = resid_kind;
+    pro.resid_tol = resid_tol;
+    pro.ep_s = d_ep_s;
+    pro.dil_ep_s = dil_ep_s;
+    if (ur != 0.0) {
+        pro.relax = 1;
+        pro.f1 = 1.0 / ur;  // F1 = ONE/UR, F2 = F1 - ONE (under_relax.f:67-68)
+        pro.f2 = pro.f1 - 1.0;
+    }
+    const Context &C = ctx();
+    return solve_with_prologue(make_dgrid(C.grid), p, cmethod ? cmethod : "RSRS", tol, itmax, d_var, d_a_m, d_b_m,
+                               opts, stats, s, mixed != 0, C.grid.minimize_dotproducts, C.grid.icheck_bicgs, pro,
+                               debug_resid, norm, resid_kind >= 0 ? resid : nullptr);
 }
 
 // Host-array drop-in for LEQ_BICGS (model/leq_bicgs.f:23-90).

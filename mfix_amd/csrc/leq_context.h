@@ -53,6 +53,8 @@ struct Shape {
 };
 
 int size_class(int64_t n3);
+void launch_resid_finish(const DGrid &g, int kind, int debug, double norm, double *partials, int64_t nblk,
+                         mfix_leq_resid *d_out, hipStream_t s);
 int default_block_size(int64_t n3, bool fp32);
 int default_lines_per_block(int64_t n3);
 double mixed_delta(int64_t n3);
@@ -158,6 +160,10 @@ struct SysDesc {
     double *x, *a, *b;            // VAR, A_M, B_M (FP64, device)
     const double *noise;
     uint64_t seed;
+    Prologue pro;                 // residual / under-relaxation in the set-up pass
+    mfix_leq_resid *resid_out = nullptr;  // host: the residual's result (pro.resid_kind >= 0)
+    int resid_debug = 0;
+    double resid_norm = 1.0;
 };
 
 // Workspace bytes of one system and the carving of that workspace.
@@ -190,6 +196,10 @@ int solve_fp64(const DGrid &g, int pc, const char *cmethod, double tol, int itma
                const mfix_leq_opts *opts, mfix_leq_stats *stats, hipStream_t s, int minimize, int icheck);
 int solve_mixed(const DGrid &g, int pc, const char *cmethod, double tol, int itmax, double *x, double *a, double *b,
                 const mfix_leq_opts *opts, mfix_leq_stats *stats, hipStream_t s);
+int solve_with_prologue(const DGrid &g, int pc, const char *cmethod, double tol, int itmax, double *x, double *a,
+                        double *b, const mfix_leq_opts *opts, mfix_leq_stats *stats, hipStream_t s, int mixed,
+                        int minimize, int icheck, const Prologue &pro, int resid_debug, double resid_norm,
+                        mfix_leq_resid *resid_out);
 // grids: one grid per item (NULL: a 3-D serial grid from each item's sizes)
 int solve_batch(int nsys, const mfix_leq_batch_item *items, mfix_leq_stats *stats, hipStream_t s,
                 const mfix_leq_grid *grids = nullptr);

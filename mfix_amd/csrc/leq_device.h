@@ -288,6 +288,61 @@ __device__ __forceinline__ float stencil_row_diff(const DGrid &g, PA a, const X 
     return stencil_combine_diff(g.do_k, a0, a1, a2, a3, a4, a5, a6, xkm, xim, xjm, xc, xjp, xip, xkp);
 }
 
+// ---- CALC_RESID rows ---------------------------------------------------------
+
+enum ResidKind : int { RESID_C = 0, RESID_S = 1, RESID_PP = 2 };
+
+// One row of CALC_RESID_C / _S / _PP (model/calc_resid.f:86-110, :280-303,
+// :440-447), serial semantics: false for a cell the sums skip (ghost layer,
+// not FLUID_AT; for S |VAR| <= tol or a dilute solids cell).  NUM1 =
+// |B - sum A*VAR| with a wall neighbour replaced by the cell itself for C
+// (EAST_OF and the like, model/set_index1a.f:93-97), DEN1 = |A0*VAR|; for PP
+// NUM1 = |B|, DEN1 = 1.
+template <typename PA, typename PV, typename PF>
+__device__ __forceinline__ bool resid_row(const DGrid &g, int kind, double tol, const double *ep_s, double dil_ep_s,
+                                          PA a, PA b, PV var, PF flag, int64_t c, double &num1, double &den1) {
+    if (c >= g.n3 || !is_interior(g, c) || !fluid_at(flag[c])) return false;  // IS_ON_myPE_wobnd, FLUID_AT
+    if (kind == RESID_PP) {
+        num1 = fabs(b[c]);
+        den1 = 1.0;
+        return true;
+    }
+    const double vc = var[c];
+    if (kind == RESID_S) {
+        if (!(fabs(vc) > tol)) return false;
+        if (ep_s && to_global(ep_s)[c] <= dil_ep_s) return false;
+    }
+    const Nb n = neighbours_fast(g, c);
+    int64_t e = n.ip, w = n.im, no = n.jp, so = n.jm, t = n.kp, bo = n.km;
+    if (kind == RESID_C) {
+        e = wall_at(flag[e]) ? c : e;
+        w = wall_at(flag[w]) ? c : w;
+        no = wall_at(flag[no]) ? c : no;
+        so = wall_at(flag[so]) ? c : so;
+        t = wall_at(flag[t]) ? c : t;
+        bo = wall_at(flag[bo]) ? c : bo;
+    }
+    const int64_t n3 = g.n3;
+    // B - (A0 V + AE VE + AW VW + AN VN + AS VS), then the top and bottom
+    // pair (calc_resid.f:94-103), in that order
+    double r = b[c] - (a[3 * n3 + c] * vc + a[4 * n3 + c] * var[e] + a[2 * n3 + c] * var[w] +
+                       a[5 * n3 + c] * var[no] + a[1 * n3 + c] * var[so]);
+    if (g.do_k) r = r - (a[6 * n3 + c] * var[t] + a[0 * n3 + c] * var[bo]);
+    num1 = fabs(r);
+    den1 = fabs(a[3 * n3 + c] * vc);
+    return true;
+}
+
+// The larger residual candidate (value, 0-based cell); equal values keep
+// the first cell, as the reference's scan with a strict '>' does.
+__device__ __forceinline__ void resid_better(double &v, double &i, double ov, double oi) {
+    if (ov > v || (ov == v && oi < i)) {
+        v = ov;
+        i = oi;
+    }
+}
+constexpr int kResidVals = 5;  // partial slots per block: NUM, DEN, fluid cells, best value, best cell
+
 // ---- launch shape ---------------------------------------------------------
 //
 // A launch has nblk logical blocks of BS threads; logical block b owns the

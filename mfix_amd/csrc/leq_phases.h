@@ -22,6 +22,20 @@
 
 namespace mfix {
 
+// Work the set-up pass does on a row before it scales it
+// (mfix_leq_resid_relax_bicgs_dev): the CALC_RESID row of the equation as
+// passed (partials of NUM, DEN, fluid cells and the largest NUM1 per logical
+// block), then UNDER_RELAX_S.
+struct Prologue {
+    const int *flag = nullptr;  // FLAG; null: no prologue
+    int resid_kind = -1;        // RESID_C / RESID_S / RESID_PP; -1: no residual
+    double resid_tol = 0.0, dil_ep_s = 0.0;
+    const double *ep_s = nullptr;
+    double *rpart = nullptr;    // kResidVals x P.nblk partial slots
+    int relax = 0;              // UNDER_RELAX_S: A0 *= f1, B += A0 VAR f2 (f1 = 1/UR, f2 = f1 - 1)
+    double f1 = 1.0, f2 = 0.0;
+};
+
 // One system of a launch.  T is the Krylov storage type (double: plain FP64
 // solve; float: inner solve of the mixed path).  The FP64 arrays (matrix,
 // rhs, solution buffers, true residual) sit beside the T ones; for a plain
@@ -44,6 +58,7 @@ struct Sys {
     double *a64, *b64, *xa, *xb, *r64;
     const double *noise;   // optional U[0,1) array for Rtilde
     uint64_t seed;
+    Prologue pro;          // residual and under-relaxation in the set-up pass
 };
 
 // Map entry -> (system, logical block); returns from the kernel for padding.
@@ -91,8 +106,34 @@ __global__ void __launch_bounds__(BS) k_setup_residual(const Sys<T> *__restrict_
     double v[1] = {0.0};
     const int64_t n3 = g.n3;
     int zero = 0;
-    const Operand<double, OP_PLAIN, gptr<const double>> X{to_global((const double *)S.xa), nullptr};
+    const gptr<const double> xv = to_global((const double *)S.xa);
+    const Operand<double, OP_PLAIN, gptr<const double>> X{xv, nullptr};
+    // prologue (uniform over the block: one system)
+    const Prologue &pr = S.pro;
+    const bool do_res = pr.flag && pr.resid_kind >= 0, do_rel = pr.flag && pr.relax;
+    const gptr<const int> flag = to_global(pr.flag);
+    const double rinit = pr.resid_kind == RESID_PP ? -1.0 : 0.0;  // RESID_IJK scan start (calc_resid.f:114, :423)
+    double rs[3] = {0.0, 0.0, 0.0}, bv = rinit, bi = 1.0e300;
     FOR_CELLS(BS, P, blk, c) {
+        if (do_res) {
+            // CALC_RESID row of the equation as passed
+            double num1, den1;
+            if (resid_row(g, pr.resid_kind, pr.resid_tol, pr.ep_s, pr.dil_ep_s, a, b, xv, flag, c, num1, den1)) {
+                rs[0] += num1;
+                rs[1] += den1;
+                rs[2] += 1.0;
+                if (num1 > rinit) resid_better(bv, bi, num1, (double)c);
+            }
+        }
+        if (do_rel) {
+            // UNDER_RELAX_S (under_relax.f:64-76), as k_under_relax_s
+            const double ap = a[3 * n3 + c], bc = b[c], vr = xv[c];
+            const double na = ap * pr.f1, nb = bc + ap * vr * pr.f2;
+            if (fluid_at(flag[c]) && ap != -1.0) {
+                a[3 * n3 + c] = na;
+                b[c] = nb;
+            }
+        }
         double amax = 0.0;
 #pragma unroll
         for (int p = 0; p < 7; ++p) amax = fmax(amax, fabs(a[p * n3 + c]));
@@ -110,6 +151,33 @@ __global__ void __launch_bounds__(BS) k_setup_residual(const Sys<T> *__restrict_
         if (g.dot_all || n.interior) v[0] += rr * rr;
     }
     if (zero) atomicOr((int *)&to_global(S.st)->zero_row, 1);
+    if (do_res) {
+        // the block's residual partials: sums in block order, the largest
+        // NUM1 by wave shuffles then the waves in order (k_resid's reduction)
+        __shared__ double lr[(BS / 64) * 3], lbv[BS / 64], lbi[BS / 64];
+        block_sum<BS, 3>(rs, lr);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const double ov = __shfl_down(bv, off, 64), oi = __shfl_down(bi, off, 64);
+            resid_better(bv, bi, ov, oi);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            lbv[threadIdx.x >> 6] = bv;
+            lbi[threadIdx.x >> 6] = bi;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double bvv = lbv[0], bii = lbi[0];
+            for (int u = 1; u < BS / 64; ++u) resid_better(bvv, bii, lbv[u], lbi[u]);
+            auto *__restrict__ rp = to_global(pr.rpart);
+            const int64_t nb = P.nblk;
+            rp[0 * nb + blk] = rs[0];
+            rp[1 * nb + blk] = rs[1];
+            rp[2 * nb + blk] = rs[2];
+            rp[3 * nb + blk] = bvv;
+            rp[4 * nb + blk] = bii;
+        }
+    }
     block_partial<BS, 1>(v, to_global(S.partials), P.nblk, blk, lds);
 }
 

@@ -11,14 +11,11 @@
 
 namespace mfix {
 
-enum ResidKind : int { RESID_C = 0, RESID_S = 1, RESID_PP = 2 };
-
 constexpr double kUndefined = 9.87654321e31;   // UNDEFINED, model/param1_mod.f:18
 constexpr double kLargeNumber = 1.0e32;        // LARGE_NUMBER, model/param1_mod.f:23
 constexpr int kResidBlock = 256;
 constexpr int kResidCells = 8;                 // cells per thread: 2048 per block and partial
 constexpr int kResidFinish = 1024;
-constexpr int kResidVals = 5;                  // num, den, ncells, best value, best index
 
 struct ResidArgs {
     DGrid g;
@@ -31,14 +28,7 @@ struct ResidArgs {
     mfix_leq_resid *out;
 };
 
-// The larger candidate (value, 0-based cell); equal values keep the first
-// cell, as the reference's scan with a strict '>' does.
-__device__ __forceinline__ void better(double &v, double &i, double ov, double oi) {
-    if (ov > v || (ov == v && oi < i)) {
-        v = ov;
-        i = oi;
-    }
-}
+__device__ __forceinline__ void better(double &v, double &i, double ov, double oi) { resid_better(v, i, ov, oi); }
 
 __global__ void __launch_bounds__(kResidBlock) k_resid(ResidArgs R) {
     __shared__ double lds[(kResidBlock / 64) * 3];
@@ -57,38 +47,8 @@ __global__ void __launch_bounds__(kResidBlock) k_resid(ResidArgs R) {
     const int64_t c0 = (int64_t)blockIdx.x * (kResidBlock * kResidCells) + threadIdx.x;
     for (int q = 0; q < kResidCells; ++q) {
         const int64_t c = c0 + (int64_t)q * kResidBlock;   // increasing per thread
-        if (c >= g.n3 || !is_interior(g, c) || !fluid_at(flag[c])) continue;   // IS_ON_myPE_wobnd, FLUID_AT
         double num1, den1;
-        if (R.kind == RESID_PP) {
-            num1 = fabs(b[c]);
-            den1 = 1.0;
-        } else {
-            const double vc = var[c];
-            if (R.kind == RESID_S) {
-                if (!(fabs(vc) > R.tol)) continue;
-                if (R.ep_s && to_global(R.ep_s)[c] <= R.dil_ep_s) continue;
-            }
-            const Nb n = neighbours(g, c);
-            int64_t e = n.ip, w = n.im, no = n.jp, so = n.jm, t = n.kp, bo = n.km;
-            if (R.kind == RESID_C) {
-                // EAST_OF etc.: a wall neighbour is replaced by the cell
-                // itself (model/set_index1a.f:93-97 and the like)
-                e = wall_at(flag[e]) ? c : e;
-                w = wall_at(flag[w]) ? c : w;
-                no = wall_at(flag[no]) ? c : no;
-                so = wall_at(flag[so]) ? c : so;
-                t = wall_at(flag[t]) ? c : t;
-                bo = wall_at(flag[bo]) ? c : bo;
-            }
-            const int64_t n3 = g.n3;
-            // B - (A0 V + AE VE + AW VW + AN VN + AS VS), then the top and
-            // bottom pair (calc_resid.f:94-103), in that order
-            num1 = b[c] - (a[3 * n3 + c] * vc + a[4 * n3 + c] * var[e] + a[2 * n3 + c] * var[w] +
-                           a[5 * n3 + c] * var[no] + a[1 * n3 + c] * var[so]);
-            if (g.do_k) num1 = num1 - (a[6 * n3 + c] * var[t] + a[0 * n3 + c] * var[bo]);
-            num1 = fabs(num1);
-            den1 = fabs(a[3 * n3 + c] * vc);
-        }
+        if (!resid_row(g, R.kind, R.tol, R.ep_s, R.dil_ep_s, a, b, var, flag, c, num1, den1)) continue;
         sums[0] += num1;
         sums[1] += den1;
         sums[2] += 1.0;
@@ -132,11 +92,27 @@ __global__ void __launch_bounds__(kResidFinish) k_resid_finish(ResidArgs R) {
     double bv = init, bi = none;
     const auto *__restrict__ p = to_global((const double *)R.partials);
     const int64_t nb = R.nblk;
-    for (int64_t q = threadIdx.x; q < nb; q += kResidFinish) {
-        sums[0] += p[0 * nb + q];
-        sums[1] += p[1 * nb + q];
-        sums[2] += p[2 * nb + q];
-        better(bv, bi, p[3 * nb + q], p[4 * nb + q]);
+    // partials tid, tid + kResidFinish, ... in that order, the loads of
+    // eight of them issued before the first add (a solver's set-up pass
+    // leaves one partial per logical block: 67K at 256^3 with 256-cell
+    // blocks, which a rolled loop would wait on one round trip at a time)
+    constexpr int U = 8;
+    for (int64_t q0 = threadIdx.x; q0 < nb; q0 += (int64_t)U * kResidFinish) {
+        double pv[U][kResidVals];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t q = q0 + (int64_t)u * kResidFinish;
+#pragma unroll
+            for (int f = 0; f < kResidVals; ++f) pv[u][f] = q < nb ? p[f * nb + q] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (q0 + (int64_t)u * kResidFinish < nb) {
+                sums[0] += pv[u][0];
+                sums[1] += pv[u][1];
+                sums[2] += pv[u][2];
+                better(bv, bi, pv[u][3], pv[u][4]);
+            }
     }
     block_sum<kResidFinish, 3>(sums, lds);
 #pragma unroll
@@ -214,6 +190,22 @@ static int run_resid(ResidArgs R, mfix_leq_resid *out, hipStream_t s) {
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e == hipSuccess) e = hipGetLastError();
     return e == hipSuccess ? 0 : hip_fail(e, "residual");
+}
+
+// The finish of residual partials a solver's set-up pass wrote (kResidVals
+// x nblk slots, one per logical block): the same fixed-order sums and
+// normalisation, into *d_out on the device.
+void launch_resid_finish(const DGrid &g, int kind, int debug, double norm, double *partials, int64_t nblk,
+                         mfix_leq_resid *d_out, hipStream_t s) {
+    ResidArgs R = {};
+    R.g = g;
+    R.kind = kind;
+    R.debug = debug != 0;
+    R.norm = norm;
+    R.partials = partials;
+    R.nblk = nblk;
+    R.out = d_out;
+    hipLaunchKernelGGL(k_resid_finish, dim3(1), dim3(kResidFinish), 0, s, R);
 }
 
 static int resid_ready(hipStream_t *s, void *stream) {

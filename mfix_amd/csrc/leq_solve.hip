@@ -491,6 +491,7 @@ void sys_carve(Sys<T> &S, const SysDesc &d, const Part &P, char *base) {
     S.delta = d.delta;
     S.max_updates = d.max_updates;
     S.unit_diag = ctx().unit_diag;
+    S.pro = d.pro;
     S.r = (T *)take(n3 * sizeof(T));
     S.rt = (T *)take(n3 * sizeof(T));
     S.p = (T *)take(n3 * sizeof(T));
@@ -1006,6 +1007,16 @@ static int solve_one(Group<T> &G, const SysDesc &d, const Shape &sh, mfix_leq_st
     G.bs = sh.bs;
     G.mixed = mixed;
     sys_carve<T>(G.sys[0], d, P, (char *)C.ws.ptr);
+    // residual of the equation as passed, formed by the set-up pass: its
+    // partials and result in the context's partials buffer
+    const bool resid = d.pro.flag && d.pro.resid_kind >= 0 && d.resid_out;
+    mfix_leq_resid *d_rout = nullptr;
+    if (resid) {
+        const size_t pb = sizeof(double) * kResidVals * (size_t)P.nblk;
+        if ((rc = C.partials.ensure(pb + 256 + sizeof(mfix_leq_resid)))) return rc;
+        G.sys[0].pro.rpart = (double *)C.partials.ptr;
+        d_rout = (mfix_leq_resid *)(((uintptr_t)C.partials.ptr + pb + 255) & ~(uintptr_t)255);
+    }
     if ((rc = upload_table<T>(G, s))) return rc;
     if ((rc = build_maps<T>(G, std::vector<int>{0}, G.iter, s))) return rc;
     hipEvent_t e0, e1;
@@ -1013,6 +1024,8 @@ static int solve_one(Group<T> &G, const SysDesc &d, const Shape &sh, mfix_leq_st
     (void)hipEventCreate(&e1);
     (void)hipEventRecord(e0, s);
     enqueue_setup<T>(G, G.iter, s);
+    if (resid) launch_resid_finish(d.g, d.pro.resid_kind, d.resid_debug, d.resid_norm, G.sys[0].pro.rpart, P.nblk,
+                                   d_rout, s);
     const bool graphs = C.use_graphs && d.g.n3 <= ((int64_t)1 << 22);
     rc = run_group<T>(G, d.itmax, s, graphs);
     (void)hipEventRecord(e1, s);
@@ -1030,6 +1043,11 @@ static int solve_one(Group<T> &G, const SysDesc &d, const Shape &sh, mfix_leq_st
     // returned at once (the solve was done)
     C.prof[0].collect(h->k1_launches);
     C.prof[1].collect(h->line_launches);
+    if (resid) {
+        hipError_t e = hipMemcpyAsync(d.resid_out, d_rout, sizeof(mfix_leq_resid), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return hip_fail(e, "residual copy");
+    }
     if (mixed && h->x_cur == 1) {
         hipError_t e = hipMemcpyAsync(d.x, G.sys[0].xb, sizeof(double) * d.g.n3, hipMemcpyDeviceToDevice, s);
         if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -1074,6 +1092,24 @@ int solve_mixed(const DGrid &g, int pc, const char *cmethod, double tol, int itm
     const double delta = opts && opts->inner_tol > 0.0 ? opts->inner_tol : mixed_delta(g.n3);
     const SysDesc d = make_desc(g, pc, cmethod, tol, itmax, x, a, b, opts, 0, 1, delta);
     return solve_one<float>(ctx().g32, d, ctx().shape_for(g.n3, true), stats, s);
+}
+
+// A solve whose set-up pass first forms the CALC_RESID row and applies
+// UNDER_RELAX_S (pro), FP64 or mixed; the residual's result goes to
+// *resid_out.
+int solve_with_prologue(const DGrid &g, int pc, const char *cmethod, double tol, int itmax, double *x, double *a,
+                        double *b, const mfix_leq_opts *opts, mfix_leq_stats *stats, hipStream_t s, int mixed,
+                        int minimize, int icheck, const Prologue &pro, int resid_debug, double resid_norm,
+                        mfix_leq_resid *resid_out) {
+    const double delta = mixed ? (opts && opts->inner_tol > 0.0 ? opts->inner_tol : mixed_delta(g.n3)) : 0.0;
+    SysDesc d = mixed ? make_desc(g, pc, cmethod, tol, itmax, x, a, b, opts, 0, 1, delta)
+                      : make_desc(g, pc, cmethod, tol, itmax, x, a, b, opts, minimize, icheck, 0.0);
+    d.pro = pro;
+    d.resid_out = resid_out;
+    d.resid_debug = resid_debug;
+    d.resid_norm = resid_norm;
+    if (mixed) return solve_one<float>(ctx().g32, d, ctx().shape_for(g.n3, true), stats, s);
+    return solve_one<double>(ctx().g64, d, ctx().shape_for(g.n3, false), stats, s);
 }
 
 // ---- batched solves ---------------------------------------------------------------

@@ -369,6 +369,33 @@ def calc_resid_dev(kind: str, *, b: "DeviceArray", flag: "DeviceArray", var: "De
     return out.as_dict()
 
 
+_RESID_KINDS = {"C": 0, "S": 1, "PP": 2}
+
+
+def resid_relax_bicgs_dev(kind: str | None, var: "DeviceArray", a_m: "DeviceArray", b_m: "DeviceArray",
+                          flag: "DeviceArray", ur: float = 0.0, *, resid_tol: float = 0.0,
+                          ep_s: "DeviceArray | None" = None, dil_ep_s: float = 1.0e-4, norm: float = 1.0,
+                          debug: bool = True, pc: str = "DIAG", cmethod: str = "RSRS", tol: float = 1e-10,
+                          itmax: int = 500, mixed: bool = False, noise: "DeviceArray | None" = None,
+                          seed: int = 0) -> tuple[dict | None, "SolveResult"]:
+    """CALC_RESID (kind "C", "S", "PP" or None), UNDER_RELAX_S (ur, 0 = none)
+    and LEQ_BICGS of one equation, the residual and the relaxation done by
+    the solver's set-up pass.  Returns (the residual's dict or None, the
+    solve's result)."""
+    out = Resid()
+    st = Stats()
+    o = _opts(noise, seed)
+    k = -1 if kind is None else _RESID_KINDS[kind]
+    rc = _lib_().mfix_leq_resid_relax_bicgs_dev(
+        k, flag.ptr, float(resid_tol), None if ep_s is None else ep_s.ptr, float(dil_ep_s), float(norm), int(debug),
+        C.byref(out), float(ur), cmethod.encode().ljust(4)[:4], float(tol), pc.encode().ljust(4)[:4], int(itmax),
+        int(mixed), var.ptr, a_m.ptr, b_m.ptr, C.byref(o), C.byref(st), None)
+    if rc == -101:
+        raise SolverAbort("preconditioner option not found - check mfix.dat and readme")
+    check(rc, "resid_relax_bicgs_dev")
+    return (None if k < 0 else out.as_dict()), SolveResult(st.ier, st.as_dict())
+
+
 # ---- the gas pressure-correction equation on the device ---------------------
 # (INIT_AB_M + CONV_PP_G + SOURCE_PP_G, SOLVE_PP_G: model/conv_pp_g.f,
 #  model/source_pp_g.f, model/solve_pp_g.f)
