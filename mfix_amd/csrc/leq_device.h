@@ -298,16 +298,16 @@ enum ResidKind : int { RESID_C = 0, RESID_S = 1, RESID_PP = 2 };
 // |B - sum A*VAR| with a wall neighbour replaced by the cell itself for C
 // (EAST_OF and the like, model/set_index1a.f:93-97), DEN1 = |A0*VAR|; for PP
 // NUM1 = |B|, DEN1 = 1.
-template <typename PA, typename PV, typename PF>
-__device__ __forceinline__ bool resid_row(const DGrid &g, int kind, double tol, const double *ep_s, double dil_ep_s,
-                                          PA a, PA b, PV var, PF flag, int64_t c, double &num1, double &den1) {
+template <typename PV, typename PF>
+__device__ __forceinline__ bool resid_row_vals(const DGrid &g, int kind, double tol, const double *ep_s,
+                                               double dil_ep_s, const double (&ar)[7], double bc, double vc, PV var,
+                                               PF flag, int64_t c, double &num1, double &den1) {
     if (c >= g.n3 || !is_interior(g, c) || !fluid_at(flag[c])) return false;  // IS_ON_myPE_wobnd, FLUID_AT
     if (kind == RESID_PP) {
-        num1 = fabs(b[c]);
+        num1 = fabs(bc);
         den1 = 1.0;
         return true;
     }
-    const double vc = var[c];
     if (kind == RESID_S) {
         if (!(fabs(vc) > tol)) return false;
         if (ep_s && to_global(ep_s)[c] <= dil_ep_s) return false;
@@ -322,15 +322,29 @@ __device__ __forceinline__ bool resid_row(const DGrid &g, int kind, double tol, 
         t = wall_at(flag[t]) ? c : t;
         bo = wall_at(flag[bo]) ? c : bo;
     }
-    const int64_t n3 = g.n3;
     // B - (A0 V + AE VE + AW VW + AN VN + AS VS), then the top and bottom
     // pair (calc_resid.f:94-103), in that order
-    double r = b[c] - (a[3 * n3 + c] * vc + a[4 * n3 + c] * var[e] + a[2 * n3 + c] * var[w] +
-                       a[5 * n3 + c] * var[no] + a[1 * n3 + c] * var[so]);
-    if (g.do_k) r = r - (a[6 * n3 + c] * var[t] + a[0 * n3 + c] * var[bo]);
+    double r = bc - (ar[3] * vc + ar[4] * var[e] + ar[2] * var[w] + ar[5] * var[no] + ar[1] * var[so]);
+    if (g.do_k) r = r - (ar[6] * var[t] + ar[0] * var[bo]);
     num1 = fabs(r);
-    den1 = fabs(a[3 * n3 + c] * vc);
+    den1 = fabs(ar[3] * vc);
     return true;
+}
+
+// resid_row_vals with the row, B and VAR loaded here (PP reads only B).
+template <typename PA, typename PV, typename PF>
+__device__ __forceinline__ bool resid_row(const DGrid &g, int kind, double tol, const double *ep_s, double dil_ep_s,
+                                          PA a, PA b, PV var, PF flag, int64_t c, double &num1, double &den1) {
+    if (c >= g.n3 || !is_interior(g, c) || !fluid_at(flag[c])) return false;
+    double ar[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double vc = 0.0;
+    if (kind != RESID_PP) {
+        const int64_t n3 = g.n3;
+#pragma unroll
+        for (int p = 0; p < 7; ++p) ar[p] = a[p * n3 + c];
+        vc = var[c];
+    }
+    return resid_row_vals(g, kind, tol, ep_s, dil_ep_s, ar, b[c], vc, var, flag, c, num1, den1);
 }
 
 // The larger residual candidate (value, 0-based cell); equal values keep

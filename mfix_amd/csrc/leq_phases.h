@@ -115,10 +115,17 @@ __global__ void __launch_bounds__(BS) k_setup_residual(const Sys<T> *__restrict_
     const double rinit = pr.resid_kind == RESID_PP ? -1.0 : 0.0;  // RESID_IJK scan start (calc_resid.f:114, :423)
     double rs[3] = {0.0, 0.0, 0.0}, bv = rinit, bi = 1.0e300;
     FOR_CELLS(BS, P, blk, c) {
+        // the row, B and VAR(c) in registers for all three steps
+        double ar[7];
+#pragma unroll
+        for (int p = 0; p < 7; ++p) ar[p] = a[p * n3 + c];
+        double bc = b[c];
+        const double vc = xv[c];
         if (do_res) {
             // CALC_RESID row of the equation as passed
             double num1, den1;
-            if (resid_row(g, pr.resid_kind, pr.resid_tol, pr.ep_s, pr.dil_ep_s, a, b, xv, flag, c, num1, den1)) {
+            if (resid_row_vals(g, pr.resid_kind, pr.resid_tol, pr.ep_s, pr.dil_ep_s, ar, bc, vc, xv, flag, c, num1,
+                               den1)) {
                 rs[0] += num1;
                 rs[1] += den1;
                 rs[2] += 1.0;
@@ -127,26 +134,32 @@ __global__ void __launch_bounds__(BS) k_setup_residual(const Sys<T> *__restrict_
         }
         if (do_rel) {
             // UNDER_RELAX_S (under_relax.f:64-76), as k_under_relax_s
-            const double ap = a[3 * n3 + c], bc = b[c], vr = xv[c];
-            const double na = ap * pr.f1, nb = bc + ap * vr * pr.f2;
+            const double ap = ar[3];
+            const double na = ap * pr.f1, nb = bc + ap * vc * pr.f2;
             if (fluid_at(flag[c]) && ap != -1.0) {
-                a[3 * n3 + c] = na;
-                b[c] = nb;
+                ar[3] = na;
+                bc = nb;
             }
         }
         double amax = 0.0;
 #pragma unroll
-        for (int p = 0; p < 7; ++p) amax = fmax(amax, fabs(a[p * n3 + c]));
+        for (int p = 0; p < 7; ++p) amax = fmax(amax, fabs(ar[p]));
         if (amax > 0.0) {
             const double oam = 1.0 / amax;
 #pragma unroll
-            for (int p = 0; p < 7; ++p) a[p * n3 + c] = a[p * n3 + c] * oam;
-            b[c] = b[c] * oam;
+            for (int p = 0; p < 7; ++p) ar[p] = ar[p] * oam;
+            bc = bc * oam;
         } else {
             zero = 1;
         }
+        if (amax > 0.0 || do_rel) {
+#pragma unroll
+            for (int p = 0; p < 7; ++p) a[p * n3 + c] = ar[p];
+            b[c] = bc;
+        }
         const Nb n = neighbours_fast(g, c);
-        const double rr = b[c] - stencil_row<double>(g, a, X, c, n);
+        const double rr = bc - stencil_combine<double>(g.do_k, ar[0], ar[1], ar[2], ar[3], ar[4], ar[5], ar[6], X(n.km),
+                                                       X(n.im), X(n.jm), vc, X(n.jp), X(n.ip), X(n.kp));
         r[c] = rr;
         if (g.dot_all || n.interior) v[0] += rr * rr;
     }
