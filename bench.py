@@ -10,8 +10,9 @@ copies of A_M/B_M already resident in HBM (the solve scales them in place).
 
 Printed: ONE JSON line with solves/s, p50/p99 solve latency, the plain FP64
 solver on the same system and the mixed result's error against it, the HBM
-roofline of the dominant kernel (HIP event times measured in this run, over
-the launches that did work) and a CPU baseline: the oracle's C/OpenMP
+roofline of the dominant kernel (kernel start and end times the dispatch
+records in HIP events, measured in this run, over the launches that did
+work) and a CPU baseline: the oracle's C/OpenMP
 restatement of the reference's LEQ_BICGS (its loop structure and serial dot
 products) timed on this host's cores on a bounded sample of the same solve.
 """
@@ -688,8 +689,9 @@ def main(argv=None):
 
     # dominant kernel: the first FP32 stencil pass k_matvec_dots4<BS, 0>
     # (V = A*(P/A0) and <Rtilde,V>, four cells per thread): 6 off-diagonal planes + row excess
-    # + operand + dot partner + result = 10*4 = 40 B/cell (HIP events in this
-    # run; A0 = -1 on every unit-scaled row of this system, so 1/A0 is not read).
+    # + operand + dot partner + result = 10*4 = 40 B/cell (start and end events
+    # recorded by each launch's dispatch in this run; A0 = -1 on every
+    # unit-scaled row of this system, so 1/A0 is not read).
     # Only launches that worked count: the ones issued after a solve was done
     # return at once and are reported apart (idle_launches).
     kern_avg_s = prof["total_ms"] * 1e-3 / max(1, prof["launches"])

@@ -36,8 +36,8 @@ struct Prof {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
     std::vector<double> pending_bytes;
     size_t pool_used = 0;
-    void begin(hipStream_t s);
-    void end(hipStream_t s, double bytes);
+    // the event pair of the next timed launch, for launch_timed (nullptrs when off)
+    std::pair<hipEvent_t, hipEvent_t> take(double bytes);
     void collect(int64_t working = -1);
 };
 constexpr int kNumProf = 2;

@@ -18,9 +18,23 @@
 // which rounds exactly like the reference's separate division pass and saves
 // two vector writes and reads per preconditioner application.
 #pragma once
+#include <hip/hip_ext.h>
+
 #include "leq_kernels.h"
 
 namespace mfix {
+
+// Launch `kernel`; given a pair of events the dispatch itself records the
+// kernel's start and end in them (hipExtLaunchKernelGGL), so a per-launch
+// time is the kernel's, without the launch latency that events recorded on
+// the stream around it would add.  No events: a plain launch (the only form
+// used inside a captured graph).
+template <typename... Args, typename... P>
+inline void launch_timed(hipEvent_t e0, hipEvent_t e1, void (*kernel)(Args...), dim3 grid, dim3 block, hipStream_t s,
+                         P... args) {
+    if (e0) hipExtLaunchKernelGGL(kernel, grid, block, 0, s, e0, e1, 0, static_cast<Args>(args)...);
+    else hipLaunchKernelGGL(kernel, grid, block, 0, s, static_cast<Args>(args)...);
+}
 
 // Work the set-up pass does on a row before it scales it
 // (mfix_leq_resid_relax_bicgs_dev): the CALC_RESID row of the equation as
@@ -1300,16 +1314,16 @@ __host__ __device__ inline int line_pcr_lpb(int var) { return 4 * (64 / line_pcr
 
 // Launch the k_line_pcr_sys variant `var` (line_pcr_variant) over `grid` blocks.
 inline void launch_line_pcr(int var, dim3 grid, hipStream_t s, const Sys<float> *tab, const int2 *map,
-                            int param, int which) {
+                            int param, int which, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
     const dim3 bs(256);
     switch (var) {
-    case 0: hipLaunchKernelGGL((k_line_pcr_sys<16, 2>), grid, bs, 0, s, tab, map, param, which); break;
-    case 1: hipLaunchKernelGGL((k_line_pcr_sys<16, 4>), grid, bs, 0, s, tab, map, param, which); break;
-    case 2: hipLaunchKernelGGL((k_line_pcr_sys<32, 4>), grid, bs, 0, s, tab, map, param, which); break;
-    case 3: hipLaunchKernelGGL((k_line_pcr_sys<32, 5>), grid, bs, 0, s, tab, map, param, which); break;
-    case 4: hipLaunchKernelGGL((k_line_pcr_sys<64, 4>), grid, bs, 0, s, tab, map, param, which); break;
-    case 5: hipLaunchKernelGGL((k_line_pcr_sys<64, 5>), grid, bs, 0, s, tab, map, param, which); break;
-    default: hipLaunchKernelGGL((k_line_pcr_sys<64, 8>), grid, bs, 0, s, tab, map, param, which); break;
+    case 0: launch_timed(e0, e1, k_line_pcr_sys<16, 2>, grid, bs, s, tab, map, param, which); break;
+    case 1: launch_timed(e0, e1, k_line_pcr_sys<16, 4>, grid, bs, s, tab, map, param, which); break;
+    case 2: launch_timed(e0, e1, k_line_pcr_sys<32, 4>, grid, bs, s, tab, map, param, which); break;
+    case 3: launch_timed(e0, e1, k_line_pcr_sys<32, 5>, grid, bs, s, tab, map, param, which); break;
+    case 4: launch_timed(e0, e1, k_line_pcr_sys<64, 4>, grid, bs, s, tab, map, param, which); break;
+    case 5: launch_timed(e0, e1, k_line_pcr_sys<64, 5>, grid, bs, s, tab, map, param, which); break;
+    default: launch_timed(e0, e1, k_line_pcr_sys<64, 8>, grid, bs, s, tab, map, param, which); break;
     }
 }
 

@@ -139,24 +139,16 @@ void Context::release_all() {
 
 // ---- per-kernel event timing ----------------------------------------------
 
-void Prof::begin(hipStream_t s) {
-    if (!on) return;
-    std::pair<hipEvent_t, hipEvent_t> ev;
-    if (pool_used < events.size()) {
-        ev = events[pool_used];
-    } else {
+std::pair<hipEvent_t, hipEvent_t> Prof::take(double bytes) {
+    if (!on) return {nullptr, nullptr};
+    if (pool_used >= events.size()) {
+        std::pair<hipEvent_t, hipEvent_t> ev;
         (void)hipEventCreate(&ev.first);
         (void)hipEventCreate(&ev.second);
         events.push_back(ev);
     }
-    (void)hipEventRecord(ev.first, s);
-}
-
-void Prof::end(hipStream_t s, double bytes) {
-    if (!on) return;
-    (void)hipEventRecord(events[pool_used].second, s);
     pending_bytes.push_back(bytes);
-    ++pool_used;
+    return events[pool_used++];
 }
 
 void Prof::collect(int64_t working) {
@@ -294,10 +286,9 @@ int launch_msolve_line(const DGrid &g, const Shape &sh, const char *cmethod, con
         // the line once
         const double cells = (double)std::min<int64_t>(nthreads, (int64_t)L.nu * L.nv) * L.n *
                              (ph.kind == PH_COLOR ? 0.5 : 1.0);
-        pf.begin(s);
-        hipLaunchKernelGGL(k_line_phase<T>, dim3(blocks ? blocks : 1), dim3(lpb), 0, s, g, ph, a, r, z, scratch, st,
-                           skip_on_sexit, fatal_flag);
-        pf.end(s, cells * (7.0 * sizeof(T) + 6.0 * sizeof(T)));
+        const auto ev = pf.take(cells * (7.0 * sizeof(T) + 6.0 * sizeof(T)));
+        launch_timed(ev.first, ev.second, k_line_phase<T>, dim3(blocks ? blocks : 1), dim3(lpb), s, g, ph, a, r, z,
+                     scratch, st, skip_on_sexit, fatal_flag);
     };
     for (int it = 0; it < 4; ++it) {
         const char ch = cmethod ? cmethod[it] : 'R';
@@ -363,11 +354,13 @@ int launch_msolve_line(const DGrid &g, const Shape &sh, const char *cmethod, con
 // nothing, for a sweep string with I/K/A passes (the caller runs
 // k_line_phase instead).
 template <typename T>
-static void launch_line_color(int lv, dim3 gs, hipStream_t s, const Sys<T> *tab, const int2 *mp, int param) {
+static void launch_line_color(int lv, dim3 gs, hipStream_t s, const Sys<T> *tab, const int2 *mp, int param,
+                              int which, std::pair<hipEvent_t, hipEvent_t> ev = {nullptr, nullptr}) {
+    const dim3 lb(kLineBlock);
     switch (lv) {
-    case 0: hipLaunchKernelGGL((k_line_color_sys<T, 64>), gs, dim3(kLineBlock), 0, s, tab, mp, param, 0); break;
-    case 1: hipLaunchKernelGGL((k_line_color_sys<T, 32>), gs, dim3(kLineBlock), 0, s, tab, mp, param, 0); break;
-    default: hipLaunchKernelGGL((k_line_color_sys<T, 16>), gs, dim3(kLineBlock), 0, s, tab, mp, param, 0); break;
+    case 0: launch_timed(ev.first, ev.second, k_line_color_sys<T, 64>, gs, lb, s, tab, mp, param, which); break;
+    case 1: launch_timed(ev.first, ev.second, k_line_color_sys<T, 32>, gs, lb, s, tab, mp, param, which); break;
+    default: launch_timed(ev.first, ev.second, k_line_color_sys<T, 16>, gs, lb, s, tab, mp, param, which); break;
     }
 }
 
@@ -416,15 +409,14 @@ int run_msolve_line_rb(const DGrid &g, const char *cmethod, const T *a, const T 
     const dim3 gs((unsigned)map.size());
     for (int q = 0; q < np; ++q)
         for (int param : {1, 0}) {
-            pf.begin(s);
+            const auto ev = pf.take(0.5 * (double)g.n3 * 13.0 * sizeof(T));
             if constexpr (sizeof(T) == sizeof(float)) {
                 const Sys<float> *tf = (const Sys<float> *)tab;
-                if (var >= 0) launch_line_pcr(var, gs, s, tf, mp, param, 0);
-                else launch_line_color<float>(lv, gs, s, tf, mp, param);
+                if (var >= 0) launch_line_pcr(var, gs, s, tf, mp, param, 0, ev.first, ev.second);
+                else launch_line_color<float>(lv, gs, s, tf, mp, param, 0, ev);
             } else {
-                launch_line_color<T>(lv, gs, s, tab, mp, param);
+                launch_line_color<T>(lv, gs, s, tab, mp, param, 0, ev);
             }
-            pf.end(s, 0.5 * (double)g.n3 * 13.0 * sizeof(T));
         }
     // the table must outlive the launches before it is rewritten
     e = hipStreamSynchronize(s);
@@ -728,22 +720,17 @@ static void line_sweeps(const Group<T> &G, const MapSet &M, int which, hipStream
             // those of every LINE system of the table (a single solve: its own)
             for (int v = 0; v < kLineLpbVariants; ++v) {
                 if (M.lmap[v][q].empty()) continue;
-                const dim3 gs((unsigned)M.lmap[v][q].size()), lb(kLineBlock);
-                pf.begin(s);
-                switch (v) {
-                case 0: hipLaunchKernelGGL((k_line_color_sys<T, 64>), gs, lb, 0, s, G.d_sys, M.d_lmap[v][q], param, which); break;
-                case 1: hipLaunchKernelGGL((k_line_color_sys<T, 32>), gs, lb, 0, s, G.d_sys, M.d_lmap[v][q], param, which); break;
-                default: hipLaunchKernelGGL((k_line_color_sys<T, 16>), gs, lb, 0, s, G.d_sys, M.d_lmap[v][q], param, which); break;
-                }
-                pf.end(s, 0.5 * M.cells * 13.0 * sizeof(T));
+                const auto ev = pf.take(0.5 * M.cells * 13.0 * sizeof(T));
+                launch_line_color<T>(v, dim3((unsigned)M.lmap[v][q].size()), s, G.d_sys, M.d_lmap[v][q], param, which,
+                                     ev);
             }
             if constexpr (sizeof(T) == sizeof(float)) {
                 for (int v = 0; v < kPcrVariants; ++v) {
                     if (M.pmap[v][q].empty()) continue;
                     const Sys<float> *tab = (const Sys<float> *)G.d_sys;
-                    pf.begin(s);
-                    launch_line_pcr(v, dim3((unsigned)M.pmap[v][q].size()), s, tab, M.d_pmap[v][q], param, which);
-                    pf.end(s, 0.5 * M.cells * 13.0 * sizeof(T));
+                    const auto ev = pf.take(0.5 * M.cells * 13.0 * sizeof(T));
+                    launch_line_pcr(v, dim3((unsigned)M.pmap[v][q].size()), s, tab, M.d_pmap[v][q], param, which,
+                                    ev.first, ev.second);
                 }
             }
         }
@@ -766,12 +753,12 @@ static void iteration_bs(const Group<T> &G, const MapSet &M, hipStream_t s) {
     const dim3 gblk(BS / Group16<T>::N);  // grouped update kernels: 16 bytes per lane and array
     hipLaunchKernelGGL((k_p_update<BS, T>), gs, gblk, 0, s, G.d_sys, M.d_smap);
     if (line) line_sweeps(G, M, 0, s);
-    pf.begin(s);
+    const auto ev = pf.take(M.k1_bytes);
     if constexpr (sizeof(T) == sizeof(float))
-        hipLaunchKernelGGL((k_matvec_dots4<BS, 0>), gs, dim3(BS / 4), 0, s, (const Sys<float> *)G.d_sys, M.d_smap);
+        launch_timed(ev.first, ev.second, k_matvec_dots4<BS, 0>, gs, dim3(BS / 4), s, (const Sys<float> *)G.d_sys,
+                     M.d_smap);
     else
-        hipLaunchKernelGGL((k_matvec_dots<BS, T, 0>), gs, blk, 0, s, G.d_sys, M.d_smap);
-    pf.end(s, M.k1_bytes);
+        launch_timed(ev.first, ev.second, k_matvec_dots<BS, T, 0>, gs, blk, s, G.d_sys, M.d_smap);
     finish<T, FIN_ALPHA, 1>(G, M, s);
     hipLaunchKernelGGL((k_s_update<BS, T>), gs, gblk, 0, s, G.d_sys, M.d_smap);
     if (line) line_sweeps(G, M, 1, s);
