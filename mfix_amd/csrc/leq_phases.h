@@ -772,14 +772,15 @@ __global__ void __launch_bounds__(kFinishBlock) k_finish_sys(const Sys<T> *__res
 // `param` of system map[b].x, numbered densely by color_line, with five
 // waves: lane l < LPB of wave 0 runs DGTSV on line l (the operations and
 // order of solve_line, so the result is bit-identical); waves 1-4 feed it.
-// Lines along J are contiguous, so a loader wave reads rows in groups: a
-// tile is two groups of RL = 512/LPB rows, loader w takes row group
-// (w-1)/2 of line half (w-1)%2, and element k of its lane is row lane % RL
-// of line (half) + lane / RL + (LPB/8) k, each load instruction fetching
-// LPB/8 lines x RL contiguous rows; it forms the right-hand side (B minus
-// the I/K neighbour terms, whose offsets are constant along a J-line) and
-// stages it in LDS, while wave 0 eliminates the previous tile: one memory
-// round trip per tile (16 rows at 64 lines per block, 32 at 32, 64 at 16).
+// Lines along J are contiguous, so a loader wave reads rows in runs: a tile
+// is RT = 1024/LPB rows of every line, loader w takes the lines of quarter
+// w-1, and element k of its lane is row lane % RT of line (quarter) +
+// lane / RT + (LPB/16) k, each load instruction fetching LPB/16 lines x RT
+// contiguous rows (16 rows, 128 bytes of FP64, per line at 64 lines per
+// block); it forms the right-hand side (B minus the I/K neighbour terms,
+// whose offsets are constant along a J-line) and stages it in LDS, while
+// wave 0 eliminates the previous tile: one memory round trip per tile (16
+// rows at 64 lines per block, 32 at 32, 64 at 16).
 // Finished rows go to scratch in [row][lane] order; for the back solve
 // waves 1-4 stage them in LDS in groups of 8 rows, two groups ahead, and
 // waves 1-2 write the solution out through LDS, coalesced.  Fewer lines per block give more blocks (128^3: 8,450 lines per
@@ -791,10 +792,10 @@ template <typename T, int LPB>
 __global__ void __launch_bounds__(320) k_line_color_sys(const Sys<T> *__restrict__ sys, const int2 *__restrict__ map,
                                                         int param, int which) {
     static_assert(LPB == 64 || LPB == 32 || LPB == 16, "lines per block");
-    constexpr int RL = 512 / LPB;  // rows per row group
+    constexpr int RL = 1024 / LPB; // rows per run a loader instruction reads of each line
     constexpr int LPI = 64 / RL;   // lines per load instruction
     constexpr int K = 4;           // tile elements per loader lane
-    constexpr int RT = 2 * RL;     // rows per elimination tile
+    constexpr int RT = RL;         // rows per elimination tile
     constexpr int RB = 8;          // rows per back-solve group
     constexpr int KS = LPB / 16;   // solution elements per lane of each storing wave
     // LDS rows padded so that a loader's staging store (RL rows x LPI lines)
@@ -818,8 +819,8 @@ __global__ void __launch_bounds__(320) k_line_color_sys(const Sys<T> *__restrict
     const int wave = threadIdx.x >> 6;  // 0: elimination and back solve; 1-4: loads (1, 2: stores)
     const int lane = threadIdx.x & 63;
     const bool mine = lane < LPB;       // wave 0: this lane solves line `lane` of the block
-    const int ro = wave >= 1 ? RL * ((wave - 1) >> 1) : 0;       // a loader's first row within a tile
-    const int lh = wave >= 1 ? (LPB / 2) * ((wave - 1) & 1) : 0;  // its first line
+    const int ro = 0;                                      // a loader's first row within a tile
+    const int lh = wave >= 1 ? (LPB / 4) * (wave - 1) : 0;  // its first line
     const int do_k = g.do_k;
     constexpr int pd = DiagPlane<T>::value;
     const gptr<const T> a = to_global(S.a);
