@@ -477,11 +477,13 @@ int mfix_leq_set_graphs(int on);
  * No counterpart in the reference (a bandwidth switch of this library). */
 int mfix_leq_set_unit_diag(int on);
 
-/* ---- timing of the dominant kernel (HIP events on the solver stream) ---- */
-/* Enable per-launch event timing of kernel `which` (0 = first stencil pass
- * of each iteration, 1 = line preconditioner colour phase); read back the
- * launch count, the summed milliseconds and the algorithmic bytes of the
- * launches that worked.  Launches of a single solve issued after it was
+/* ---- timing of the dominant kernel (HIP events of each dispatch) ------- */
+/* Enable per-launch timing of kernel `which` (0 = first stencil pass of each
+ * iteration, 1 = line preconditioner colour phase): the timed launches
+ * record their own start and end in HIP events (hipExtLaunchKernelGGL), and
+ * iterations run as direct launches instead of graph replays while it is
+ * on.  Read back the launch count, the summed milliseconds and the
+ * algorithmic bytes of the launches that worked.  Launches of a single solve issued after it was
  * done (they return at once) are counted apart: mfix_leq_prof_idle.
  * No counterpart in the reference (measurement of this library). */
 int mfix_leq_prof_enable(int which, int on);
