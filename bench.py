@@ -527,7 +527,7 @@ def bench_serve(args):
     for mixed in (True, False):
         for rate in rates:
             reqs = systems.serving_requests(rate, nreq, seed=4000)
-            with Server(64 << 30, 64) as srv:
+            with Server(64 << 30, args.slots) as srv:
                 live = {}
                 done_t = {}
                 nxt = 0
@@ -566,7 +566,7 @@ def bench_serve(args):
         "ms_per_step": 1e3 / top["solves_per_s"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
         "config": {"workload": f"{nreq} requests per rate, sides lognormal(64, 0.5) clipped to [24,160], 60% "
-                               "upwind LINE / 40% pressure DIAG, TOL 1e-8|1e-10, HBM budget 64 GiB, 64 slots",
+                               f"upwind LINE / 40% pressure DIAG, TOL 1e-8|1e-10, HBM budget 64 GiB, {args.slots} slots",
                    "p50_ms": top["p50_ms"], "p99_ms": top["p99_ms"], "rate": top["rate"], "sweep": results},
         "roofline": None,
         "cpu_baseline": None,
@@ -592,6 +592,7 @@ def main(argv=None):
                     help="first run the plain copy kernel 3x on 512 MiB (HBM counter calibration)")
     ap.add_argument("--rates", default="50,200,1000,2000", help="serve: request rates (1/s)")
     ap.add_argument("--requests", type=int, default=120, help="serve: requests per rate")
+    ap.add_argument("--slots", type=int, default=64, help="serve: server slots (systems iterated together)")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this solver has no multi-GPU path (asked for --gpus {args.gpus})",
