@@ -804,7 +804,6 @@ __global__ void __launch_bounds__(320) k_line_color_sys(const Sys<T> *__restrict
     // the rows share banks: 8- to 32-way conflicts)
     constexpr int SL = LPB + LPI, SX = LPB + 8;
     __shared__ T tl[2][4][RT][SL]; // dl, d, du, bb: RT rows x LPB lines, double buffered
-    __shared__ T xs[RB][SX];       // solution rows staged for the stores
     __shared__ int linfo[LPB];      // DGTSV INFO per line
     const int2 e = map[blockIdx.x];
     if (e.x < 0) return;
@@ -831,6 +830,8 @@ __global__ void __launch_bounds__(320) k_line_color_sys(const Sys<T> *__restrict
     const gptr<T> sc = to_global(S.line_scratch + (int64_t)e.y * 4 * n * LPB);
     const uint32_t c1 = (uint32_t)g.c1, c2 = (uint32_t)g.c2;
     const uint32_t pl = (uint32_t)n * LPB;  // finished rows: 4 arrays of n rows x LPB lanes
+    const uint32_t pb = pl * (uint32_t)sizeof(T);  // (bytes)
+    const gptr<char> scb = reinterpret_cast<gptr<char>>(sc);
     int u = 0, v = 0;
     const int vld = mine && color_line(L.nu, L.nv, do_k ? 1 : 0, param, (int64_t)e.y * LPB + lane, u, v) ? 1 : 0;
     const int rsub = lane % RL;
@@ -896,62 +897,63 @@ __global__ void __launch_bounds__(320) k_line_color_sys(const Sys<T> *__restrict
             // DGTSV elimination of this lane's line over the tile
             // (DGTSV.f:107-163, the operations of solve_line in the same order)
             const int q0 = tt * RT;
+            const int re = n - q0 < RT ? n - q0 : RT;  // rows of this tile
+            int r = 0;
+            if (tt == 0) {  // row 0 only starts the recurrence
+                d0 = tl[0][1][0][lane];
+                du0 = tl[0][2][0][lane];
+                b0 = tl[0][3][0][lane];
+                r = 1;
+            }
             // row r+1's values are read while row r is eliminated, so the LDS
             // latency stays off the serial chain of divisions
-            T ndl = tl[buf][0][0][lane], nd = tl[buf][1][0][lane], ndu = tl[buf][2][0][lane];
-            T nb = tl[buf][3][0][lane];
-            for (int r = 0; r < RT; ++r) {
-                const int q = q0 + r;
-                if (q >= n) break;
-                const T dl1 = ndl;
-                T d1 = nd, du1 = ndu, b1 = nb;
-                if (r + 1 < RT) {
-                    ndl = tl[buf][0][r + 1][lane];
-                    nd = tl[buf][1][r + 1][lane];
-                    ndu = tl[buf][2][r + 1][lane];
-                    nb = tl[buf][3][r + 1][lane];
-                }
-                if (q == 0) {
-                    d0 = d1;
-                    du0 = du1;
-                    b0 = b1;
-                    continue;
-                }
-                const int i = q - 1;
+            T ndl = tl[buf][0][r][lane], nd = tl[buf][1][r][lane], ndu = tl[buf][2][r][lane];
+            T nb = tl[buf][3][r][lane];
+            // finished row i = q0 + r - 1 of the scratch arrays, as a byte
+            // offset from the block's base (32-bit offsets, one add per row)
+            uint32_t so = ((uint32_t)(q0 + r - 1) * LPB + (uint32_t)lane) * (uint32_t)sizeof(T);
+            for (; r < re; ++r) {
+                const T dl1 = ndl, d1 = nd, du1 = ndu, b1 = nb;
+                const int rn = r + 1 < RT ? r + 1 : RT - 1;  // in the tile; unused past its end
+                ndl = tl[buf][0][rn][lane];
+                nd = tl[buf][1][rn][lane];
+                ndu = tl[buf][2][rn][lane];
+                nb = tl[buf][3][rn][lane];
+                const int i = q0 + r - 1;
+                const bool keep = fabs((double)d0) >= fabs((double)dl1);  // no interchange
                 T fd, fu, fu2, fb;
-                if (fabs((double)d0) >= fabs((double)dl1)) {
+                if (__all(keep)) {
+                    // no lane interchanges rows (every diagonally dominant line)
                     if (d0 == (T)0 && !info) info = i + 1;
                     const T fact = dl1 / d0;
-                    d1 = d1 - fact * du0;
-                    b1 = b1 - fact * b0;
                     fd = d0;
                     fu = du0;
                     fu2 = (T)0;
                     fb = b0;
+                    d0 = d1 - fact * du0;
+                    du0 = du1;
+                    b0 = b1 - fact * b0;
                 } else {
-                    const T fact = d0 / dl1;
-                    fd = dl1;
-                    T temp = d1;
-                    d1 = du0 - fact * temp;
-                    if (i < n - 2) {
-                        fu2 = du1;
-                        du1 = -fact * fu2;
-                    } else {
-                        fu2 = (T)0;
-                    }
-                    fu = temp;
-                    temp = b0;
-                    fb = b1;
-                    b1 = temp - fact * b1;
+                    // per lane, row i stays or trades places with row i+1: one
+                    // division and the same products, on selected operands
+                    if (keep && d0 == (T)0 && !info) info = i + 1;
+                    const T fact = (keep ? dl1 : d0) / (keep ? d0 : dl1);
+                    const T pm = keep ? du0 : d1;  // row i's DU after the interchange
+                    const T bm = keep ? b0 : b1;   // row i's B after the interchange
+                    const bool sh = !keep && i < n - 2;
+                    fd = keep ? d0 : dl1;
+                    fu = pm;
+                    fu2 = sh ? du1 : (T)0;
+                    fb = bm;
+                    d0 = (keep ? d1 : du0) - fact * pm;
+                    du0 = sh ? -fact * du1 : du1;
+                    b0 = (keep ? b1 : b0) - fact * bm;
                 }
-                const uint32_t o = (uint32_t)i * LPB + lane;
-                sc[o] = fd;
-                sc[pl + o] = fu;
-                sc[2 * pl + o] = fu2;
-                sc[3 * pl + o] = fb;
-                d0 = d1;
-                du0 = du1;
-                b0 = b1;
+                *reinterpret_cast<gptr<T>>(scb + so) = fd;
+                *reinterpret_cast<gptr<T>>(scb + (so + pb)) = fu;
+                *reinterpret_cast<gptr<T>>(scb + (so + 2 * pb)) = fu2;
+                *reinterpret_cast<gptr<T>>(scb + (so + 3 * pb)) = fb;
+                so += LPB * (uint32_t)sizeof(T);
             }
         }
         __syncthreads();
@@ -982,6 +984,9 @@ __global__ void __launch_bounds__(320) k_line_color_sys(const Sys<T> *__restrict
     constexpr int DB = 2;             // groups in flight per staging lane
     static_assert(2 * GE <= 2 * 4 * RT * SL, "two staged groups fit in the tile buffers");
     T(*bs)[4][RB][LPB] = reinterpret_cast<T(*)[4][RB][LPB]>(&tl[0][0][0][0]);  // slot t & 1
+    // solution rows of group t in slot t & 1, row hi - r in xs[t & 1][r]
+    T(*xs)[RB][SX] = reinterpret_cast<T(*)[RB][SX]>(&tl[0][0][0][0] + 2 * GE);
+    static_assert(2 * GE + 2 * RB * SX <= 2 * 4 * RT * SL, "staged groups and solution rows fit in the tile buffers");
     const int ngrp = (n + RB - 1) / RB;
     const bool stager = wave >= 1;
     const int fl = (wave - 1) * 64 + lane;  // 0..255 on the staging waves
@@ -1004,31 +1009,32 @@ __global__ void __launch_bounds__(320) k_line_color_sys(const Sys<T> *__restrict
         }
     };
     T x1 = mine ? b0 / d0 : (T)0, x2 = (T)0;
-    // group t: solve it from LDS (wave 0), stage group t+1 and put group
-    // t+1+DB in flight in the register set group t+1 held (waves 1-4; set
-    // (t+1) % DB, so the loop is unrolled by DB), then store its solution
-    // rows (waves 1-2)
-    auto group = [&](int t, T(&set)[FE]) __attribute__((always_inline)) {
+    // group t: solve it from LDS (wave 0) into solution slot t & 1, stage
+    // group t+1 and put group t+1+DB in flight in the register set group t+1
+    // held (waves 1-4; set (t+1) % DB, so the loop is unrolled by DB), then
+    // store its solution rows (waves 1-2).  One barrier per group: the
+    // staging of group t+2 into bs[t & 1] and the solving of group t+2 into
+    // xs[t & 1] both come after the next group's barrier, which every
+    // reader of group t's slots has passed.
+    auto group = [&](int t, T(&set)[FE], bool first) __attribute__((always_inline)) {
         const int hi = n - 1 - t * RB, lo = hi - RB + 1 > 0 ? hi - RB + 1 : 0;
+        T(&xo)[RB][SX] = xs[t & 1];
         if (wave == 0 && mine) {
+            // row hi - r; rows below lo (last group only) are zeros in LDS and
+            // give values that are neither stored nor used
 #pragma unroll
             for (int r = 0; r < RB; ++r) {
-                const int q = hi - r;
-                if (q >= lo) {
-                    T x;
-                    if (q == n - 1) {
-                        x = x1;
-                    } else if (q == n - 2) {
-                        x = (bs[t & 1][3][r][lane] - bs[t & 1][1][r][lane] * x1) / bs[t & 1][0][r][lane];
-                        x2 = x1;
-                        x1 = x;
-                    } else {
-                        x = (bs[t & 1][3][r][lane] - bs[t & 1][1][r][lane] * x1 - bs[t & 1][2][r][lane] * x2) /
-                            bs[t & 1][0][r][lane];
-                        x2 = x1;
-                        x1 = x;
-                    }
-                    xs[q - lo][lane] = x;
+                const T bq = bs[t & 1][3][r][lane], uq = bs[t & 1][1][r][lane];
+                const T u2q = bs[t & 1][2][r][lane], dq = bs[t & 1][0][r][lane];
+                if (first && r == 0) {
+                    xo[0][lane] = x1;  // row n-1: B/D of the elimination
+                } else {
+                    // row n-2 has U2 = 0 and x2 = 0 here, so this is its
+                    // (B - DU x1) / D bit for bit (DGTSV.f:233-246)
+                    const T x = (bq - uq * x1 - u2q * x2) / dq;
+                    x2 = x1;
+                    x1 = x;
+                    xo[r][lane] = x;
                 }
             }
         } else if (stager && t + 1 < ngrp) {
@@ -1039,12 +1045,11 @@ __global__ void __launch_bounds__(320) k_line_color_sys(const Sys<T> *__restrict
         if (wave == 1 || wave == 2) {
 #pragma unroll
             for (int k = 0; k < KS; ++k) {
-                const int q = lo + (lane & 7);
+                const int q = hi - (lane & 7);
                 const int j = (lane >> 3) + 8 * (KS * (wave - 1) + k);
-                if (sjc[k] != ~0u && q <= hi) var[sjc[k] + (uint32_t)q] = xs[lane & 7][j];
+                if (sjc[k] != ~0u && q >= lo) var[sjc[k] + (uint32_t)q] = xo[lane & 7][j];
             }
         }
-        __syncthreads();
     };
     if (stager) {
         issue(0, fr[0]);
@@ -1054,8 +1059,8 @@ __global__ void __launch_bounds__(320) k_line_color_sys(const Sys<T> *__restrict
     }
     __syncthreads();
     for (int t0 = 0; t0 < ngrp; t0 += DB) {
-        group(t0, fr[1]);
-        if (t0 + 1 < ngrp) group(t0 + 1, fr[0]);
+        group(t0, fr[1], t0 == 0);
+        if (t0 + 1 < ngrp) group(t0 + 1, fr[0], false);
     }
 }
 
