@@ -905,21 +905,12 @@ __global__ void __launch_bounds__(320) k_line_color_sys(const Sys<T> *__restrict
                 b0 = tl[0][3][0][lane];
                 r = 1;
             }
-            // row r+1's values are read while row r is eliminated, so the LDS
-            // latency stays off the serial chain of divisions
-            T ndl = tl[buf][0][r][lane], nd = tl[buf][1][r][lane], ndu = tl[buf][2][r][lane];
-            T nb = tl[buf][3][r][lane];
             // finished row i = q0 + r - 1 of the scratch arrays, as a byte
             // offset from the block's base (32-bit offsets, one add per row)
             uint32_t so = ((uint32_t)(q0 + r - 1) * LPB + (uint32_t)lane) * (uint32_t)sizeof(T);
-            for (; r < re; ++r) {
-                const T dl1 = ndl, d1 = nd, du1 = ndu, b1 = nb;
-                const int rn = r + 1 < RT ? r + 1 : RT - 1;  // in the tile; unused past its end
-                ndl = tl[buf][0][rn][lane];
-                nd = tl[buf][1][rn][lane];
-                ndu = tl[buf][2][rn][lane];
-                nb = tl[buf][3][rn][lane];
-                const int i = q0 + r - 1;
+            // eliminate with row i+1 = (dl1, d1, du1, b1), store finished row i
+            auto row = [&](const T dl1, const T d1, const T du1, const T b1, const int i)
+                           __attribute__((always_inline)) {
                 const bool keep = fabs((double)d0) >= fabs((double)dl1);  // no interchange
                 T fd, fu, fu2, fb;
                 if (__all(keep)) {
@@ -954,7 +945,24 @@ __global__ void __launch_bounds__(320) k_line_color_sys(const Sys<T> *__restrict
                 *reinterpret_cast<gptr<T>>(scb + (so + 2 * pb)) = fu2;
                 *reinterpret_cast<gptr<T>>(scb + (so + 3 * pb)) = fb;
                 so += LPB * (uint32_t)sizeof(T);
+            };
+            // two rows per step in two register sets, each read from LDS one
+            // row ahead of its elimination (the LDS latency stays off the
+            // serial chain of divisions, and no values move between sets)
+            T adl = tl[buf][0][r][lane], ad = tl[buf][1][r][lane], adu = tl[buf][2][r][lane];
+            T ab = tl[buf][3][r][lane];
+            for (; r + 1 < re; r += 2) {
+                const T bdl = tl[buf][0][r + 1][lane], bd = tl[buf][1][r + 1][lane];
+                const T bdu = tl[buf][2][r + 1][lane], bb = tl[buf][3][r + 1][lane];
+                row(adl, ad, adu, ab, q0 + r - 1);
+                const int r2 = r + 2 < RT ? r + 2 : RT - 1;  // in the tile; unused past its end
+                adl = tl[buf][0][r2][lane];
+                ad = tl[buf][1][r2][lane];
+                adu = tl[buf][2][r2][lane];
+                ab = tl[buf][3][r2][lane];
+                row(bdl, bd, bdu, bb, q0 + r);
             }
+            if (r < re) row(adl, ad, adu, ab, q0 + r - 1);
         }
         __syncthreads();
     }
