@@ -912,15 +912,17 @@ __global__ void __launch_bounds__(320) k_line_color_sys(const Sys<T> *__restrict
             auto row = [&](const T dl1, const T d1, const T du1, const T b1, const int i)
                            __attribute__((always_inline)) {
                 const bool keep = fabs((double)d0) >= fabs((double)dl1);  // no interchange
-                T fd, fu, fu2, fb;
+                auto put = [&](const T fd, const T fu, const T fu2, const T fb) __attribute__((always_inline)) {
+                    *reinterpret_cast<gptr<T>>(scb + so) = fd;
+                    *reinterpret_cast<gptr<T>>(scb + (so + pb)) = fu;
+                    *reinterpret_cast<gptr<T>>(scb + (so + 2 * pb)) = fu2;
+                    *reinterpret_cast<gptr<T>>(scb + (so + 3 * pb)) = fb;
+                };
                 if (__all(keep)) {
                     // no lane interchanges rows (every diagonally dominant line)
                     if (d0 == (T)0 && !info) info = i + 1;
                     const T fact = dl1 / d0;
-                    fd = d0;
-                    fu = du0;
-                    fu2 = (T)0;
-                    fb = b0;
+                    put(d0, du0, (T)0, b0);
                     d0 = d1 - fact * du0;
                     du0 = du1;
                     b0 = b1 - fact * b0;
@@ -932,18 +934,11 @@ __global__ void __launch_bounds__(320) k_line_color_sys(const Sys<T> *__restrict
                     const T pm = keep ? du0 : d1;  // row i's DU after the interchange
                     const T bm = keep ? b0 : b1;   // row i's B after the interchange
                     const bool sh = !keep && i < n - 2;
-                    fd = keep ? d0 : dl1;
-                    fu = pm;
-                    fu2 = sh ? du1 : (T)0;
-                    fb = bm;
+                    put(keep ? d0 : dl1, pm, sh ? du1 : (T)0, bm);
                     d0 = (keep ? d1 : du0) - fact * pm;
                     du0 = sh ? -fact * du1 : du1;
                     b0 = (keep ? b1 : b0) - fact * bm;
                 }
-                *reinterpret_cast<gptr<T>>(scb + so) = fd;
-                *reinterpret_cast<gptr<T>>(scb + (so + pb)) = fu;
-                *reinterpret_cast<gptr<T>>(scb + (so + 2 * pb)) = fu2;
-                *reinterpret_cast<gptr<T>>(scb + (so + 3 * pb)) = fb;
                 so += LPB * (uint32_t)sizeof(T);
             };
             // two rows per step in two register sets, each read from LDS one
