@@ -1196,18 +1196,21 @@ __global__ void __launch_bounds__(256) k_line_pcr_sys(const Sys<float> *__restri
         rd[j] = in ? ln[3 * NS + o] : 0.f;
     }
     // 1. rows 0 .. m-2 of the chunk: forward elimination carrying the
-    // coupling to the previous lane's separator, then backward substitution
+    // coupling to the previous lane's separator, then backward substitution.
+    // Reciprocals here and in the reduction are the hardware's (1 ulp, one
+    // instruction) instead of correctly rounded divisions (about ten): this
+    // FP32 solve is a preconditioner whose result the FP64 updates correct.
     float cp[M], wp[M], dp[M], U[M], V[M], W[M];
 #pragma unroll
     for (int j = 0; j < M - 1; ++j) {
         if (j < m - 1) {
             if (j == 0) {
-                const float r = 1.f / rb[0];
+                const float r = __builtin_amdgcn_rcpf(rb[0]);
                 cp[0] = rc[0] * r;
                 wp[0] = ra[0] * r;
                 dp[0] = rd[0] * r;
             } else {
-                const float r = 1.f / (rb[j] - ra[j] * cp[j - 1]);
+                const float r = __builtin_amdgcn_rcpf(rb[j] - ra[j] * cp[j - 1]);
                 cp[j] = rc[j] * r;
                 wp[j] = -ra[j] * wp[j - 1] * r;
                 dp[j] = (rd[j] - ra[j] * dp[j - 1]) * r;
@@ -1265,8 +1268,8 @@ __global__ void __launch_bounds__(256) k_line_pcr_sys(const Sys<float> *__restri
         const int sl = base + (hl ? gl - sft : gl), sr = base + (hr ? gl + sft : gl);
         const float Al = __shfl(A, sl, 64), Bl = __shfl(B, sl, 64), Cl = __shfl(Cc, sl, 64), Dl = __shfl(D, sl, 64);
         const float Ar = __shfl(A, sr, 64), Br = __shfl(B, sr, 64), Cr = __shfl(Cc, sr, 64), Dr = __shfl(D, sr, 64);
-        const float al = hl ? -A / Bl : 0.f;
-        const float ga = hr ? -Cc / Br : 0.f;
+        const float al = hl ? -A * __builtin_amdgcn_rcpf(Bl) : 0.f;
+        const float ga = hr ? -Cc * __builtin_amdgcn_rcpf(Br) : 0.f;
         const float nA = hl ? al * Al : 0.f;
         const float nC = hr ? ga * Cr : 0.f;
         const float nB = B + (hl ? al * Cl : 0.f) + (hr ? ga * Ar : 0.f);
@@ -1276,7 +1279,7 @@ __global__ void __launch_bounds__(256) k_line_pcr_sys(const Sys<float> *__restri
         Cc = nC;
         D = nD;
     }
-    const float G = D / B;
+    const float G = D * __builtin_amdgcn_rcpf(B);
     float Gp = __shfl(G, base + (gl > 0 ? gl - 1 : 0), 64);
     if (gl == 0) Gp = 0.f;
     // 3. the chunk's rows, back to LDS (this lane's own slots of the
