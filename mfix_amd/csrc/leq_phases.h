@@ -1102,11 +1102,15 @@ __host__ __device__ inline int line_lpb_variant(int lpb) { return lpb >= 64 ? 0 
 // [map[b].y*LPB, +LPB) of colour `param` (numbered densely by color_line), four
 // waves of 64/P lines.  which = 0 preconditions P into Phat, 1 S into Shat.
 template <int P, int M>
-__global__ void __launch_bounds__(256) k_line_pcr_sys(const Sys<float> *__restrict__ sys,
-                                                      const int2 *__restrict__ map, int param, int which) {
+__host__ __device__ constexpr int pcr_lds_floats() {
+    return 4 * (64 / P) * (4 * P * (M + 1) + P);  // lines per block x LS (below)
+}
+
+template <int P, int M>
+__device__ __forceinline__ void line_pcr_block(const Sys<float> *__restrict__ sys, const int2 e, int param,
+                                               int which, float *__restrict__ lds) {
     constexpr int LPW = 64 / P;
     constexpr int LPB = 4 * LPW;
-    const int2 e = map[blockIdx.x];
     if (e.x < 0) return;
     const Sys<float> &S = sys[e.x];
     auto *__restrict__ st = to_global(S.st);
@@ -1137,7 +1141,7 @@ __global__ void __launch_bounds__(256) k_line_pcr_sys(const Sys<float> *__restri
     // LDS: per line, the four coefficient arrays of NS slots; lines LS apart
     // (LS = P mod 64: the groups of a wave start in different banks)
     constexpr int NS = P * (M + 1), LS = 4 * NS + P;
-    __shared__ float lds[LPB * LS];
+    static_assert(LPB * LS == pcr_lds_floats<P, M>(), "LDS size");
     float *__restrict__ ln = lds + ((threadIdx.x >> 6) * LPW + grp) * LS;
     const int sc = m | 1;                 // chunk stride in LDS
     const float rm = 1.f / (float)m;      // (q + 0.5) * rm truncates to q / m for q < 4096, m <= 8
@@ -1305,6 +1309,39 @@ __global__ void __launch_bounds__(256) k_line_pcr_sys(const Sys<float> *__restri
         const int q = gl + P * j;
         if (q < n) var[c0 + (uint32_t)q] = ln[3 * NS + slot(q)];
     }
+}
+
+template <int P, int M>
+__global__ void __launch_bounds__(256) k_line_pcr_sys(const Sys<float> *__restrict__ sys,
+                                                      const int2 *__restrict__ map, int param, int which) {
+    __shared__ float lds[pcr_lds_floats<P, M>()];
+    line_pcr_block<P, M>(sys, map[blockIdx.x], param, which, lds);
+}
+
+// The four short-line variants (lines of up to 160 rows) of one colour in
+// one launch: a table of systems with lines of several lengths (a batch)
+// then issues one launch per colour phase instead of one per variant, and
+// the variants' blocks run side by side.  Blocks [0, n[3]) take map m[3]
+// (P 32, M 5), the next n[2] map m[2], and so on: the longest first.
+struct PcrMaps {
+    const int2 *m[4];
+    int n[4];
+};
+constexpr int kPcrSmallLds = pcr_lds_floats<32, 5>();
+static_assert(kPcrSmallLds >= pcr_lds_floats<32, 4>() && kPcrSmallLds >= pcr_lds_floats<16, 4>() &&
+                  kPcrSmallLds >= pcr_lds_floats<16, 2>(),
+              "LDS of the short-line variants");
+static __global__ void __launch_bounds__(256) k_line_pcr_short(const Sys<float> *__restrict__ sys,
+                                                               const PcrMaps pm, int param, int which) {
+    __shared__ float lds[kPcrSmallLds];
+    int b = blockIdx.x;
+    if (b < pm.n[3]) return line_pcr_block<32, 5>(sys, pm.m[3][b], param, which, lds);
+    b -= pm.n[3];
+    if (b < pm.n[2]) return line_pcr_block<32, 4>(sys, pm.m[2][b], param, which, lds);
+    b -= pm.n[2];
+    if (b < pm.n[1]) return line_pcr_block<16, 4>(sys, pm.m[1][b], param, which, lds);
+    b -= pm.n[1];
+    if (b < pm.n[0]) line_pcr_block<16, 2>(sys, pm.m[0][b], param, which, lds);
 }
 
 // Variant of k_line_pcr_sys for a line of n rows: lanes per line P and most

@@ -725,9 +725,23 @@ static void line_sweeps(const Group<T> &G, const MapSet &M, int which, hipStream
                                      ev);
             }
             if constexpr (sizeof(T) == sizeof(float)) {
-                for (int v = 0; v < kPcrVariants; ++v) {
+                const Sys<float> *tab = (const Sys<float> *)G.d_sys;
+                // the short-line variants in one launch when more than one holds systems
+                int nshort = 0;
+                PcrMaps pm{};
+                for (int v = 0; v < 4; ++v) {
+                    pm.m[v] = M.d_pmap[v][q];
+                    pm.n[v] = (int)M.pmap[v][q].size();
+                    nshort += pm.n[v] > 0 ? 1 : 0;
+                }
+                if (nshort > 1) {
+                    const auto ev = pf.take(0.5 * M.cells * 13.0 * sizeof(T));
+                    launch_timed(ev.first, ev.second, k_line_pcr_short,
+                                 dim3((unsigned)(pm.n[0] + pm.n[1] + pm.n[2] + pm.n[3])), dim3(256), s, tab, pm,
+                                 param, which);
+                }
+                for (int v = nshort > 1 ? 4 : 0; v < kPcrVariants; ++v) {
                     if (M.pmap[v][q].empty()) continue;
-                    const Sys<float> *tab = (const Sys<float> *)G.d_sys;
                     const auto ev = pf.take(0.5 * M.cells * 13.0 * sizeof(T));
                     launch_line_pcr(v, dim3((unsigned)M.pmap[v][q].size()), s, tab, M.d_pmap[v][q], param, which,
                                     ev.first, ev.second);
